@@ -1,0 +1,88 @@
+"""Build and load libchfft_hip.so, the HIP (gfx950) engine behind the C ABI in
+include/chfft_hip.h. There is no CPU fallback: if the library is missing or the
+GPU is absent, calls fail loudly."""
+import ctypes
+import os
+import subprocess
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+_ROOT = os.path.dirname(_PKG)
+LIB_PATH = os.path.join(_PKG, "libchfft_hip.so")
+SOURCES = ["csrc/chfft_kernels.hip", "csrc/chfft_plan.hip"]
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # no packed FP32 instructions (v_pk_*_f32) in any kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+OK, ERR_INVALID_LENGTH, ERR_INVALID_ARGUMENT, ERR_UNSUPPORTED, ERR_HIP, ERR_OOM = range(6)
+F32, F64 = 0, 1
+METHODS = {
+    "forward": 0, "forward0": 1, "forwardu": 2, "forwardn": 3,
+    "backward": 4, "backward0": 5, "backwardu": 6, "backwardn": 7,
+}
+DCT_II, DCT_III = 2, 3
+
+_LIB = None
+
+
+def build(verbose=False):
+    """Compile the HIP sources into LIB_PATH (cross-compiles without a GPU)."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    cmd = [hipcc] + HIPCC_FLAGS + [os.path.join(_PKG, s) for s in SOURCES] + ["-o", LIB_PATH + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stderr[-4000:])
+    os.replace(LIB_PATH + ".tmp", LIB_PATH)
+    if verbose:
+        print("built", LIB_PATH)
+    return LIB_PATH
+
+
+# Every symbol include/chfft_hip.h declares, with its ctypes signature.
+def _signatures():
+    vp, sz, ci = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    return {
+        "chfft_plan_c2c_1d": ([pp, sz, sz, ci], ci),
+        "chfft_plan_r2c_1d": ([pp, sz, sz, ci], ci),
+        "chfft_plan_c2c_2d": ([pp, sz, sz, sz, ci], ci),
+        "chfft_plan_dct_1d": ([pp, ci, sz, ci, sz, ci], ci),
+        "chfft_plan_mdct_1d": ([pp, sz, vp, sz, sz, ci], ci),
+        "chfft_plan_destroy": ([vp], ci),
+        "chfft_exec": ([vp, vp, vp, ci, vp], ci),
+        "chfft_exec_host": ([vp, vp, vp, ci], ci),
+        "chfft_plan_io_bytes": ([vp, ci, ctypes.POINTER(sz), ctypes.POINTER(sz)], ci),
+        "chfft_plan_scratch_bytes": ([vp], sz),
+        "chfft_plan_describe": ([vp, ctypes.c_char_p, sz], ci),
+        "chfft_status_string": ([ci], ctypes.c_char_p),
+    }
+
+
+def lib():
+    """Load the built library (does not build it: build() first)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP extension not built: {LIB_PATH} is missing (run chfft_amd.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _signatures().items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _LIB = L
+    return _LIB
+
+
+class ChfftError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = lib().chfft_status_string(status).decode()
+        super().__init__(f"{what}: {msg}" if what else msg)
+
+
+def check(status, what=""):
+    if status != OK:
+        if status == ERR_INVALID_LENGTH:
+            raise ValueError("invalid length")
+        raise ChfftError(status, what)

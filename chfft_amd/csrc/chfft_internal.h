@@ -1,0 +1,96 @@
+// Launch descriptors shared by the host planner (chfft_plan.hip) and the
+// device kernels (chfft_kernels.hip). Plain data only: a KParams is passed by
+// value as the kernel argument.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+namespace chfft {
+
+constexpr int kMaxPass = 24;
+// Register-staged LDS passes: each thread holds at most kEmax complex values of
+// its butterflies between the LDS read and the LDS write of one pass.
+constexpr int kEmax = 16;
+// Workgroup size limits of the fused kernel (FP32 / FP64).
+constexpr int kMaxThreadsF32 = 1024;
+constexpr int kMaxThreadsF64 = 512;
+// LDS per workgroup on gfx950.
+constexpr size_t kLdsBytes = 160 * 1024;
+
+// One radix-R Stockham pass over a length-L sequence resident in LDS.
+// p = product of the radices already applied; tws = L / (p * R) is the step of
+// the twiddle index j*k*tws into the W_L table.
+struct PassDesc {
+  int radix;
+  int p;
+  int tws;
+};
+
+struct SeqDesc {
+  int L;
+  int npass;
+  PassDesc pass[kMaxPass];
+  const void* tw;  // W_L^m, m in [0, L), in the kernel's precision
+};
+
+// Element idx of transform g lives at
+//   (g / ncols) * bdist + (g % ncols) * cstride + idx * estride
+// in units of the element type (complex for complex data, scalar for real).
+struct IoDesc {
+  long long bdist;
+  long long cstride;
+  long long estride;
+  int ncols;
+};
+
+// Load-stage operators (what the first LDS image is made from).
+enum LoadOp : int {
+  LD_PLAIN = 0,     // complex element (also real pairs viewed as complex)
+  LD_FS_CHIRP = 1,  // Bluestein column pass: chirp multiply + zero padding
+  LD_R2C_BACK = 2,  // RFft1D backward pre-pass (Hermitian un-split)
+  LD_DCT2 = 3,      // DCT-II even/odd packing
+  LD_DCT3 = 4,      // DCT-III pre-twiddle + split
+  LD_MDCT = 5,      // MDCT window + fold + pre-twiddle
+  LD_IMDCT = 6,     // IMDCT pre-twiddle
+};
+
+// Store-stage operators (what is written from the final LDS image).
+enum StoreOp : int {
+  ST_PLAIN = 0,
+  ST_TW = 1,           // four-step inter-pass twiddle W_N^(c*k)
+  ST_FS_DECHIRP = 2,   // Bluestein last column pass: de-chirp, crop to N
+  ST_R2C = 3,          // RFft1D forward split
+  ST_DCT2 = 4,
+  ST_DCT3 = 5,
+  ST_MDCT = 6,
+  ST_IMDCT = 7,
+};
+
+struct KParams {
+  const void* src;
+  void* dst;
+  int G;       // transforms in this launch
+  int tpb;     // transforms per workgroup
+  int Lfft;    // FFT length held in LDS (M for a fused Bluestein)
+  int Lp;      // LDS stride between transforms (>= Lfft)
+  int L;       // logical inner length (== Lfft unless fused Bluestein)
+  int colmajor_ld;  // 1: consecutive threads take consecutive transforms on load
+  int colmajor_st;  // same for the store stage
+  int conj_in;      // conjugate after the load operator (inverse via conj trick)
+  int conj_out;     // conjugate before the store operator
+  int has_mid;      // Bluestein spectrum product + second pass sequence
+  int has_chirp;    // fused Bluestein chirp (load) and de-chirp (before store)
+  SeqDesc seq;
+  IoDesc in, out;
+  double scale;     // user scaling, applied by the store operator
+  double ortho;     // DCT element-0 scaler
+  long long N;      // logical length the operator works on
+  long long twN;    // modulus of the four-step twiddle index
+  long long kc, ke; // spectrum index = c * kc + idx * ke
+  const void* chirp;
+  const void* kspec;
+  const void* tab_a;
+  const void* tab_b;
+};
+
+}  // namespace chfft

@@ -1,0 +1,1021 @@
+// Host side of the transform engine: the GPU plan builder and the C ABI
+// declared in include/chfft_hip.h.
+//
+// The plan builder replaces chfft's CFft1D::setup (cfft1d.rs:103-206) and the
+// constructors of the derived types. A plan is a short list of kernel launches
+// (chfft_kernels.hip) plus device tables generated on the host in long double
+// and rounded once to the plan's precision:
+//   * length fits one workgroup's LDS  -> one fused launch (Stockham passes,
+//     or a fused Bluestein when the length is not 7-smooth);
+//   * longer 7-smooth lengths          -> four-step: column pass with the
+//     inter-pass twiddle, then row pass with a transposed store (2 launches);
+//   * longer non-7-smooth lengths      -> Bluestein over a four-step M-point
+//     FFT (3 launches: chirp + columns, rows + spectrum product + inverse rows,
+//     inverse columns + de-chirp).
+// The MAX_PRIME = 7 rule and the Bluestein padding M = 2 * next_pow2(N) follow
+// cfft1d.rs:104 and cfft1d.rs:114.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/chfft_hip.h"
+#include "chfft_internal.h"
+
+namespace chfft {
+
+hipError_t launch_fused(bool f64, int ld, int st, const KParams& P, dim3 grid, dim3 block, size_t lds,
+                        hipStream_t stream);
+hipError_t set_fused_lds_limits();
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
+
+namespace {
+
+struct Status {
+  int code;
+};
+
+// ------------------------------------------------------------------ device memory
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int dev_alloc(DevBuf& b, size_t bytes) {
+  if (b.p && b.bytes >= bytes) return CHFFT_OK;
+  if (b.p) {
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  if (bytes == 0) return CHFFT_OK;
+  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    return CHFFT_ERR_OUT_OF_MEMORY;
+  }
+  b.bytes = bytes;
+  return CHFFT_OK;
+}
+
+// ------------------------------------------------------------------ host tables
+using ld_t = long double;
+const ld_t kPi = 3.141592653589793238462643383279502884L;
+
+// exp(-2 pi i num / den) with the fraction reduced exactly in integers first.
+void unit_root(unsigned long long num, unsigned long long den, ld_t* re, ld_t* im) {
+  num %= den;
+  ld_t th = -2.0L * kPi * (ld_t)num / (ld_t)den;
+  *re = cosl(th);
+  *im = sinl(th);
+}
+
+struct HostTable {
+  std::vector<ld_t> v;  // interleaved re, im (or plain reals)
+};
+
+// A host table rounded once to the plan's precision and uploaded.
+int upload(const std::vector<ld_t>& v, bool f64, DevBuf& dst) {
+  size_t n = v.size();
+  size_t bytes = n * (f64 ? 8 : 4);
+  int rc = dev_alloc(dst, bytes);
+  if (rc) return rc;
+  if (f64) {
+    std::vector<double> h(n);
+    for (size_t i = 0; i < n; i++) h[i] = (double)v[i];
+    if (hipMemcpy(dst.p, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return CHFFT_ERR_HIP;
+  } else {
+    std::vector<float> h(n);
+    for (size_t i = 0; i < n; i++) h[i] = (float)v[i];
+    if (hipMemcpy(dst.p, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return CHFFT_ERR_HIP;
+  }
+  return CHFFT_OK;
+}
+
+std::vector<ld_t> roots_table(size_t L) {  // W_L^m, m in [0, L)
+  std::vector<ld_t> v(2 * L);
+  for (size_t m = 0; m < L; m++) unit_root(m, L, &v[2 * m], &v[2 * m + 1]);
+  return v;
+}
+
+// Bluestein chirp c_n = exp(-pi i (n^2 mod 2N) / N), n in [0, N); the index is
+// formed in exact 64-bit integers (cfft1d.rs:123-128, chirpz.rs:97-101).
+std::vector<ld_t> chirp_table(size_t N) {
+  std::vector<ld_t> v(2 * N);
+  const unsigned long long d = 2ull * N;
+  for (size_t n = 0; n < N; n++) {
+    unsigned long long sq = (unsigned long long)(((unsigned __int128)n * n) % d);
+    unit_root(sq, d, &v[2 * n], &v[2 * n + 1]);
+  }
+  return v;
+}
+
+// In-place radix-2 FFT of a power-of-two length in double precision with
+// long-double twiddles (used once per Bluestein plan for the kernel spectrum).
+void host_fft_pow2(std::vector<double>& re, std::vector<double>& im) {
+  size_t n = re.size();
+  for (size_t i = 1, j = 0; i < n; i++) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      std::swap(re[i], re[j]);
+      std::swap(im[i], im[j]);
+    }
+  }
+  std::vector<double> wr(n / 2 + 1), wi(n / 2 + 1);
+  for (size_t k = 0; k < n / 2; k++) {
+    ld_t a, b;
+    unit_root(k, n, &a, &b);
+    wr[k] = (double)a;
+    wi[k] = (double)b;
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    size_t step = n / len, half = len / 2;
+    for (size_t s = 0; s < n; s += len)
+      for (size_t k = 0; k < half; k++) {
+        double c = wr[k * step], d = wi[k * step];
+        size_t a = s + k, b = a + half;
+        double xr = re[b] * c - im[b] * d, xi = re[b] * d + im[b] * c;
+        re[b] = re[a] - xr;
+        im[b] = im[a] - xi;
+        re[a] += xr;
+        im[a] += xi;
+      }
+  }
+}
+
+// Kernel spectrum K'[k] = FFT_M(rot)[k] / M with rot[i] = conj(c_i) for i < N,
+// 0 for N <= i <= M/2, rot[M - i] mirrored above M/2 (cfft1d.rs:120-140, 179;
+// the 1/M of chirpz.rs:42-46 is folded in).
+std::vector<ld_t> bluestein_spectrum(size_t N, size_t M) {
+  std::vector<ld_t> c = chirp_table(N);
+  std::vector<double> re(M, 0.0), im(M, 0.0);
+  for (size_t i = 0; i < N; i++) {
+    re[i] = (double)c[2 * i];
+    im[i] = -(double)c[2 * i + 1];
+  }
+  for (size_t i = M / 2 + 1; i < M; i++) {
+    re[i] = re[M - i];
+    im[i] = im[M - i];
+  }
+  host_fft_pow2(re, im);
+  std::vector<ld_t> v(2 * M);
+  for (size_t k = 0; k < M; k++) {
+    v[2 * k] = (ld_t)re[k] / (ld_t)M;
+    v[2 * k + 1] = (ld_t)im[k] / (ld_t)M;
+  }
+  return v;
+}
+
+// ------------------------------------------------------------------ factorization
+bool smooth7(size_t n) {
+  if (n == 0) return false;
+  for (size_t p : {2, 3, 5, 7})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+size_t next_pow2(size_t n) {
+  size_t p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+// Radix schedule for a 7-smooth L: as few passes as the radix set allows
+// (16, 8, 4, 2 for the power of two; 3, 5, 7 for the odd part).
+std::vector<int> radices_for(size_t L) {
+  std::vector<int> r;
+  int a = 0;
+  while (L % 2 == 0) {
+    L /= 2;
+    a++;
+  }
+  int k16 = a / 4, rem = a % 4;
+  if (rem == 1 && k16 > 0) {
+    k16--;
+    r.push_back(8);
+    r.push_back(4);
+  } else if (rem == 1) {
+    r.push_back(2);
+  } else if (rem == 2) {
+    r.push_back(4);
+  } else if (rem == 3) {
+    r.push_back(8);
+  }
+  for (int i = 0; i < k16; i++) r.insert(r.begin(), 16);
+  for (int p : {5, 3, 7})
+    while (L % p == 0) {
+      L /= p;
+      r.push_back(p);
+    }
+  return r;
+}
+
+size_t esz_of(bool f64) { return f64 ? 16 : 8; }
+
+int max_threads(bool f64) { return f64 ? kMaxThreadsF64 : kMaxThreadsF32; }
+
+// Largest sequence one workgroup holds: registers (kEmax per thread) and LDS.
+size_t lds_cap(bool f64) {
+  return std::min<size_t>((size_t)kEmax * max_threads(f64), kLdsBytes / esz_of(f64));
+}
+
+// ------------------------------------------------------------------ launch geometry
+struct Geometry {
+  int tpb, Lp, block;
+  size_t lds;
+};
+
+Geometry choose_geometry(int Lfft, long long G, bool colmajor, bool f64) {
+  const size_t cap = lds_cap(f64);
+  const size_t esz = esz_of(f64);
+  int tpb_max = 1;
+  if ((size_t)(Lfft + 1) * 2 <= cap) tpb_max = (int)(cap / (size_t)(Lfft + 1));
+  int tpb;
+  if (colmajor) {
+    tpb = std::min(tpb_max, 16);
+  } else {
+    tpb = std::max(1, 2048 / Lfft);
+    tpb = std::min(tpb, tpb_max);
+    while (tpb > 1 && (G + tpb - 1) / tpb < 1024) tpb >>= 1;
+  }
+  tpb = std::max(tpb, 1);
+  Geometry g;
+  g.tpb = tpb;
+  g.Lp = Lfft + (tpb > 1 ? 1 : 0);
+  long long elems = (long long)tpb * Lfft;
+  int block = (int)((elems + 7) / 8);
+  block = (block + 63) / 64 * 64;
+  int minb = (int)((elems + kEmax - 1) / kEmax);
+  minb = (minb + 63) / 64 * 64;
+  block = std::max(block, minb);
+  block = std::max(64, std::min(max_threads(f64), block));
+  g.block = block;
+  g.lds = (size_t)tpb * g.Lp * esz;
+  return g;
+}
+
+// ------------------------------------------------------------------ plan objects
+enum Kind { K_C2C = 0, K_R2C = 1, K_C2C2D = 2, K_DCT = 3, K_MDCT = 4 };
+enum Slot { S_IN = 0, S_OUT = 1, S_SCRATCH = 2 };
+
+struct Launch {
+  int ld, st;
+  KParams P;
+  dim3 grid, block;
+  size_t lds;
+  int src, dst;      // Slot
+  int scaled;        // receives the method's scale
+  double scale_mul;  // extra factor (RFft backward: 2)
+  std::string what;
+};
+
+// A fused sequence (one workgroup per tpb transforms) for a logical inner
+// length L: Stockham passes over L, or a fused Bluestein over M.
+struct Inner {
+  size_t L = 0, Lfft = 0;
+  bool bluestein = false;
+  SeqDesc seq{};
+  DevBuf tw, chirp, kspec;
+};
+
+int build_seq(size_t L, bool f64, SeqDesc& seq, DevBuf& tw) {
+  std::vector<int> rad = radices_for(L);
+  if ((int)rad.size() > kMaxPass) return CHFFT_ERR_UNSUPPORTED;
+  seq.L = (int)L;
+  seq.npass = (int)rad.size();
+  int p = 1;
+  for (size_t q = 0; q < rad.size(); q++) {
+    seq.pass[q].radix = rad[q];
+    seq.pass[q].p = p;
+    seq.pass[q].tws = (int)(L / ((size_t)p * rad[q]));
+    p *= rad[q];
+  }
+  int rc = upload(roots_table(L), f64, tw);
+  seq.tw = tw.p;
+  return rc;
+}
+
+// Can a length-L inner transform be fused into one workgroup?
+bool fusable(size_t L, bool f64) {
+  const size_t cap = lds_cap(f64);
+  if (smooth7(L)) return L <= cap;
+  return 2 * next_pow2(L) <= cap;
+}
+
+int build_inner(size_t L, bool f64, Inner& in) {
+  in.L = L;
+  if (smooth7(L)) {
+    in.Lfft = L;
+    in.bluestein = false;
+    return build_seq(L, f64, in.seq, in.tw);
+  }
+  size_t M = 2 * next_pow2(L);
+  in.Lfft = M;
+  in.bluestein = true;
+  int rc = build_seq(M, f64, in.seq, in.tw);
+  if (rc) return rc;
+  if ((rc = upload(chirp_table(L), f64, in.chirp))) return rc;
+  return upload(bluestein_spectrum(L, M), f64, in.kspec);
+}
+
+void fill_inner(KParams& P, const Inner& in) {
+  P.Lfft = (int)in.Lfft;
+  P.L = (int)in.L;
+  P.seq = in.seq;
+  P.has_mid = in.bluestein ? 1 : 0;
+  P.has_chirp = in.bluestein ? 1 : 0;
+  P.chirp = in.chirp.p;
+  P.kspec = in.kspec.p;
+  P.kc = 0;
+  P.ke = 1;
+}
+
+IoDesc io(long long bdist, long long cstride, long long estride, int ncols) {
+  IoDesc d;
+  d.bdist = bdist;
+  d.cstride = cstride;
+  d.estride = estride;
+  d.ncols = ncols;
+  return d;
+}
+
+std::string seq_str(const SeqDesc& s) {
+  std::string r = "L=" + std::to_string(s.L) + " radix[";
+  for (int i = 0; i < s.npass; i++) r += (i ? "," : "") + std::to_string(s.pass[i].radix);
+  return r + "]";
+}
+
+}  // namespace
+
+// Complex-to-complex engine for one length (used by CFft1D, both CFft2D axes).
+struct C2CEngine {
+  size_t n = 0;
+  bool f64 = false;
+  int mode = 0;  // 0 copy (n == 1), 1 fused, 2 four-step 7-smooth, 3 four-step Bluestein
+  Inner inner;   // mode 1
+  size_t n1 = 0, n2 = 0;
+  SeqDesc s1{}, s2{};
+  DevBuf tw1, tw2, twN, chirp, kspec;
+  size_t M = 0;
+
+  int init(size_t len, bool use_f64) {
+    n = len;
+    f64 = use_f64;
+    if (len == 1) {
+      mode = 0;
+      return CHFFT_OK;
+    }
+    if (fusable(len, f64)) {
+      mode = 1;
+      return build_inner(len, f64, inner);
+    }
+    size_t total = smooth7(len) ? len : 2 * next_pow2(len);
+    mode = smooth7(len) ? 2 : 3;
+    M = total;
+    // most balanced split total = n1 * n2 with n1 <= n2, both fusable with
+    // several transforms per workgroup
+    size_t best1 = 0;
+    for (size_t d = 1; d * d <= total; d++)
+      if (total % d == 0 && smooth7(d) && smooth7(total / d)) best1 = d;
+    if (best1 == 0) return CHFFT_ERR_UNSUPPORTED;
+    n1 = best1;
+    n2 = total / best1;
+    if (n2 * 4 > lds_cap(f64)) return CHFFT_ERR_UNSUPPORTED;
+    int rc;
+    if ((rc = build_seq(n1, f64, s1, tw1))) return rc;
+    if ((rc = build_seq(n2, f64, s2, tw2))) return rc;
+    if ((rc = upload(roots_table(total), f64, twN))) return rc;
+    if (mode == 3) {
+      if ((rc = upload(chirp_table(len), f64, chirp))) return rc;
+      if ((rc = upload(bluestein_spectrum(len, total), f64, kspec))) return rc;
+    }
+    return CHFFT_OK;
+  }
+
+  size_t scratch_elems(size_t batch) const { return (mode == 2 ? n : mode == 3 ? M : 0) * batch; }
+
+  // Appends the launches for `batch` transforms; `inverse` selects the
+  // conjugation pair; src/dst slots as given.
+  void emit(std::vector<Launch>& out, size_t batch, bool inverse, int src, int dst, long long dist_in,
+            long long dist_out) const {
+    const int ci = inverse ? 1 : 0;
+    if (mode == 0) {
+      Launch l{};
+      l.ld = -1;  // copy
+      l.src = src;
+      l.dst = dst;
+      l.P.G = (int)batch;
+      l.what = "copy (length 1)";
+      out.push_back(l);
+      return;
+    }
+    if (mode == 1) {
+      Launch l{};
+      l.ld = LD_PLAIN;
+      l.st = ST_PLAIN;
+      fill_inner(l.P, inner);
+      l.P.G = (int)batch;
+      l.P.in = io(dist_in, 0, 1, 1);
+      l.P.out = io(dist_out, 0, 1, 1);
+      l.P.conj_in = ci;
+      l.P.conj_out = ci;
+      Geometry g = choose_geometry((int)inner.Lfft, (long long)batch, false, f64);
+      finish(l, g, src, dst, 1);
+      l.what = std::string(inner.bluestein ? "fused bluestein N=" + std::to_string(n) + " M " : "fused ") +
+               seq_str(inner.seq);
+      out.push_back(l);
+      return;
+    }
+    if (mode == 2) {
+      // K1: columns (length n1, stride n2) + twiddle W_n^(n2*k1) -> scratch
+      Launch a{};
+      a.ld = LD_PLAIN;
+      a.st = ST_TW;
+      plain_seq(a.P, s1, n1);
+      a.P.G = (int)(batch * n2);
+      a.P.in = io(dist_in, 1, (long long)n2, (int)n2);
+      a.P.out = io((long long)n, 1, (long long)n2, (int)n2);
+      a.P.colmajor_ld = a.P.colmajor_st = 1;
+      a.P.conj_in = ci;
+      a.P.twN = (long long)n;
+      a.P.tab_a = twN.p;
+      finish(a, choose_geometry((int)n1, a.P.G, true, f64), src, S_SCRATCH, 0);
+      a.what = "four-step columns " + seq_str(s1) + " + twiddle";
+      out.push_back(a);
+      // K2: rows (length n2) -> out[k1 + n1*k2]
+      Launch b{};
+      b.ld = LD_PLAIN;
+      b.st = ST_PLAIN;
+      plain_seq(b.P, s2, n2);
+      b.P.G = (int)(batch * n1);
+      b.P.in = io((long long)n, (long long)n2, 1, (int)n1);
+      b.P.out = io(dist_out, 1, (long long)n1, (int)n1);
+      b.P.colmajor_st = 1;
+      b.P.conj_out = ci;
+      finish(b, choose_geometry((int)n2, b.P.G, true, f64), S_SCRATCH, dst, 1);
+      b.what = "four-step rows " + seq_str(s2) + " + transpose";
+      out.push_back(b);
+      return;
+    }
+    // mode 3: Bluestein over a four-step M = n1 * n2
+    Launch a{};
+    a.ld = LD_FS_CHIRP;
+    a.st = ST_TW;
+    plain_seq(a.P, s1, n1);
+    a.P.G = (int)(batch * n2);
+    a.P.N = (long long)n;
+    a.P.chirp = chirp.p;
+    a.P.in = io(dist_in, 1, (long long)n2, (int)n2);
+    a.P.out = io((long long)M, 1, (long long)n2, (int)n2);
+    a.P.colmajor_ld = a.P.colmajor_st = 1;
+    a.P.conj_in = ci;
+    a.P.twN = (long long)M;
+    a.P.tab_a = twN.p;
+    finish(a, choose_geometry((int)n1, a.P.G, true, f64), src, S_SCRATCH, 0);
+    a.what = "bluestein N=" + std::to_string(n) + " chirp + columns " + seq_str(s1) + " + twiddle";
+    out.push_back(a);
+    Launch b{};
+    b.ld = LD_PLAIN;
+    b.st = ST_TW;
+    plain_seq(b.P, s2, n2);
+    b.P.G = (int)(batch * n1);
+    b.P.in = io((long long)M, (long long)n2, 1, (int)n1);
+    b.P.out = io((long long)M, (long long)n2, 1, (int)n1);
+    b.P.has_mid = 1;
+    b.P.kspec = kspec.p;
+    b.P.kc = 1;
+    b.P.ke = (long long)n1;
+    b.P.twN = (long long)M;
+    b.P.tab_a = twN.p;
+    finish(b, choose_geometry((int)n2, b.P.G, false, f64), S_SCRATCH, S_SCRATCH, 0);
+    b.what = "bluestein rows " + seq_str(s2) + " x spectrum, inverse rows + twiddle";
+    out.push_back(b);
+    Launch c{};
+    c.ld = LD_PLAIN;
+    c.st = ST_FS_DECHIRP;
+    plain_seq(c.P, s1, n1);
+    c.P.G = (int)(batch * n2);
+    c.P.N = (long long)n;
+    c.P.chirp = chirp.p;
+    c.P.in = io((long long)M, 1, (long long)n2, (int)n2);
+    c.P.out = io(dist_out, 1, (long long)n2, (int)n2);
+    c.P.colmajor_ld = c.P.colmajor_st = 1;
+    c.P.conj_out = ci;
+    finish(c, choose_geometry((int)n1, c.P.G, true, f64), S_SCRATCH, dst, 1);
+    c.what = "bluestein inverse columns " + seq_str(s1) + " + de-chirp";
+    out.push_back(c);
+  }
+
+  static void plain_seq(KParams& P, const SeqDesc& s, size_t L) {
+    P.seq = s;
+    P.Lfft = (int)L;
+    P.L = (int)L;
+  }
+
+  static void finish(Launch& l, const Geometry& g, int src, int dst, int scaled) {
+    l.P.tpb = g.tpb;
+    l.P.Lp = g.Lp;
+    l.grid = dim3((unsigned)((l.P.G + g.tpb - 1) / g.tpb));
+    l.block = dim3((unsigned)g.block);
+    l.lds = g.lds;
+    l.src = src;
+    l.dst = dst;
+    l.scaled = scaled;
+    l.scale_mul = 1.0;
+  }
+};
+
+}  // namespace chfft
+
+using namespace chfft;
+
+struct chfft_plan_s {
+  Kind kind = K_C2C;
+  bool f64 = false;
+  size_t n = 0, m = 0, batch = 0, frame_stride = 0;
+  int dct_type = 0;
+  bool ortho = false;
+  C2CEngine rows, cols;  // c2c / 2-D
+  Inner inner;           // r2c / dct / mdct
+  DevBuf tab_a, tab_b, tab_c;
+  DevBuf scratch;
+  DevBuf stage_in, stage_out;
+  std::vector<Launch> fwd, bwd;
+};
+
+namespace chfft {
+namespace {
+
+std::once_flag g_lds_once;
+hipError_t g_lds_err = hipSuccess;
+
+int ensure_runtime() {
+  std::call_once(g_lds_once, [] { g_lds_err = set_fused_lds_limits(); });
+  return g_lds_err == hipSuccess ? CHFFT_OK : CHFFT_ERR_HIP;
+}
+
+int check_prec(int precision) {
+  return (precision == CHFFT_F32 || precision == CHFFT_F64) ? CHFFT_OK : CHFFT_ERR_INVALID_ARGUMENT;
+}
+
+// Fused single-launch recipe for r2c / dct / mdct around plan->inner.
+Launch fused_op(const chfft_plan_s* p, int ld, int st, bool inner_inverse, long long in_dist, long long out_dist,
+                int G) {
+  Launch l{};
+  l.ld = ld;
+  l.st = st;
+  fill_inner(l.P, p->inner);
+  l.P.G = G;
+  l.P.in = io(in_dist, 0, 1, 1);
+  l.P.out = io(out_dist, 0, 1, 1);
+  l.P.conj_in = l.P.conj_out = inner_inverse ? 1 : 0;
+  Geometry g = choose_geometry((int)p->inner.Lfft, G, false, p->f64);
+  C2CEngine::finish(l, g, S_IN, S_OUT, 1);
+  l.what = std::string(p->inner.bluestein ? "fused bluestein " : "fused ") + seq_str(p->inner.seq);
+  return l;
+}
+
+int finish_plan(chfft_plan_s* p, chfft_plan* out) {
+  size_t need = 0;
+  if (p->kind == K_C2C) need = p->rows.scratch_elems(p->batch);
+  if (p->kind == K_C2C2D) need = p->rows.scratch_elems(p->batch * p->m);
+  int rc = dev_alloc(p->scratch, need * esz_of(p->f64));
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return CHFFT_OK;
+}
+
+}  // namespace
+}  // namespace chfft
+
+extern "C" {
+
+const char* chfft_status_string(int status) {
+  switch (status) {
+    case CHFFT_OK: return "ok";
+    case CHFFT_ERR_INVALID_LENGTH: return "invalid length";
+    case CHFFT_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case CHFFT_ERR_UNSUPPORTED: return "unsupported";
+    case CHFFT_ERR_HIP: return "HIP runtime error";
+    case CHFFT_ERR_OUT_OF_MEMORY: return "out of device memory";
+  }
+  return "unknown status";
+}
+
+int chfft_plan_c2c_1d(chfft_plan* plan, size_t len, size_t batch, int precision) {
+  if (!plan) return CHFFT_ERR_INVALID_ARGUMENT;
+  *plan = nullptr;
+  if (check_prec(precision)) return CHFFT_ERR_INVALID_ARGUMENT;
+  if (len == 0) return CHFFT_ERR_INVALID_LENGTH;
+  if (batch == 0 || batch * len > (size_t)1 << 31) return CHFFT_ERR_INVALID_ARGUMENT;
+  int rc = ensure_runtime();
+  if (rc) return rc;
+  chfft_plan_s* p = new chfft_plan_s();
+  p->kind = K_C2C;
+  p->f64 = precision == CHFFT_F64;
+  p->n = len;
+  p->batch = batch;
+  if ((rc = p->rows.init(len, p->f64))) {
+    delete p;
+    return rc;
+  }
+  p->rows.emit(p->fwd, batch, false, S_IN, S_OUT, (long long)len, (long long)len);
+  p->rows.emit(p->bwd, batch, true, S_IN, S_OUT, (long long)len, (long long)len);
+  return finish_plan(p, plan);
+}
+
+int chfft_plan_c2c_2d(chfft_plan* plan, size_t len_m, size_t len_n, size_t batch, int precision) {
+  if (!plan) return CHFFT_ERR_INVALID_ARGUMENT;
+  *plan = nullptr;
+  if (check_prec(precision)) return CHFFT_ERR_INVALID_ARGUMENT;
+  if (len_m == 0 || len_n == 0) return CHFFT_ERR_INVALID_LENGTH;
+  if (batch == 0 || batch * len_m * len_n > (size_t)1 << 31) return CHFFT_ERR_INVALID_ARGUMENT;
+  int rc = ensure_runtime();
+  if (rc) return rc;
+  chfft_plan_s* p = new chfft_plan_s();
+  p->kind = K_C2C2D;
+  p->f64 = precision == CHFFT_F64;
+  p->m = len_m;
+  p->n = len_n;
+  p->batch = batch;
+  if ((rc = p->rows.init(len_n, p->f64))) {
+    delete p;
+    return rc;
+  }
+  if (len_m > 1 && !fusable(len_m, p->f64)) {
+    delete p;
+    return CHFFT_ERR_UNSUPPORTED;
+  }
+  if (len_m > 1 && (rc = build_inner(len_m, p->f64, p->inner))) {
+    delete p;
+    return rc;
+  }
+  for (int dir = 0; dir < 2; dir++) {
+    std::vector<Launch>& L = dir ? p->bwd : p->fwd;
+    // rows: batch * m transforms of length n (cfft2d.rs:340-348)
+    p->rows.emit(L, batch * len_m, dir == 1, S_IN, S_OUT, (long long)len_n, (long long)len_n);
+    for (auto& l : L) l.scaled = 0;
+    // columns: batch * n transforms of length m, stride n, in place on out
+    if (len_m > 1) {
+      Launch c{};
+      c.ld = LD_PLAIN;
+      c.st = ST_PLAIN;
+      fill_inner(c.P, p->inner);
+      c.P.G = (int)(batch * len_n);
+      c.P.in = io((long long)(len_m * len_n), 1, (long long)len_n, (int)len_n);
+      c.P.out = c.P.in;
+      c.P.colmajor_ld = c.P.colmajor_st = 1;
+      c.P.conj_out = dir;
+      // an inverse row pass leaves its output conjugated; a length-1 row
+      // (plain copy) does not
+      c.P.conj_in = p->rows.mode == 0 ? dir : 0;
+      Geometry g = choose_geometry((int)p->inner.Lfft, c.P.G, true, p->f64);
+      C2CEngine::finish(c, g, S_OUT, S_OUT, 1);
+      c.what = "2-D columns " + seq_str(p->inner.seq);
+      L.push_back(c);
+    } else {
+      // a single row: undo the conjugation and scale with a plain pass
+      Launch& last = L.back();
+      last.scaled = 1;
+    }
+  }
+  // row pass of an inverse: conjugate on load only, keep conjugated output
+  for (auto& l : p->bwd)
+    if (l.what.rfind("2-D columns", 0) != 0 && len_m > 1) l.P.conj_out = 0;
+  return finish_plan(p, plan);
+}
+
+int chfft_plan_r2c_1d(chfft_plan* plan, size_t len, size_t batch, int precision) {
+  if (!plan) return CHFFT_ERR_INVALID_ARGUMENT;
+  *plan = nullptr;
+  if (check_prec(precision)) return CHFFT_ERR_INVALID_ARGUMENT;
+  if (len == 0 || (len & 1)) return CHFFT_ERR_INVALID_LENGTH;  // rfft1d.rs:74-76
+  if (batch == 0 || batch * len > (size_t)1 << 31) return CHFFT_ERR_INVALID_ARGUMENT;
+  int rc = ensure_runtime();
+  if (rc) return rc;
+  const bool f64 = precision == CHFFT_F64;
+  const size_t h = len / 2;
+  if (!fusable(h, f64)) return CHFFT_ERR_UNSUPPORTED;
+  chfft_plan_s* p = new chfft_plan_s();
+  p->kind = K_R2C;
+  p->f64 = f64;
+  p->n = len;
+  p->batch = batch;
+  if ((rc = build_inner(h, f64, p->inner))) {
+    delete p;
+    return rc;
+  }
+  // coef_i = (1 + i W_N^i) / 2, bcoef_i = (1 - i W_N^-i) / 2 (rfft1d.rs:44-65)
+  std::vector<ld_t> co(2 * h), bc(2 * h);
+  for (size_t i = 0; i < h; i++) {
+    ld_t c, s;
+    unit_root(i, len, &c, &s);  // W_N^i = c + i s
+    co[2 * i] = (1.0L - s) / 2.0L;
+    co[2 * i + 1] = c / 2.0L;
+    // W_N^-i = c - i s ; i W^-i = s + i c
+    bc[2 * i] = (1.0L - s) / 2.0L;
+    bc[2 * i + 1] = -c / 2.0L;
+  }
+  if ((rc = upload(co, f64, p->tab_a)) || (rc = upload(bc, f64, p->tab_b))) {
+    delete p;
+    return rc;
+  }
+  Launch f = fused_op(p, LD_PLAIN, ST_R2C, false, (long long)h, (long long)(h + 1), (int)batch);
+  f.P.N = (long long)len;
+  f.P.tab_a = p->tab_a.p;
+  p->fwd.push_back(f);
+  Launch b = fused_op(p, LD_R2C_BACK, ST_PLAIN, true, (long long)(h + 1), (long long)h, (int)batch);
+  b.P.N = (long long)len;
+  b.P.tab_a = p->tab_b.p;
+  b.scale_mul = 2.0;  // rfft1d.rs:161
+  p->bwd.push_back(b);
+  return finish_plan(p, plan);
+}
+
+int chfft_plan_dct_1d(chfft_plan* plan, int dct_type, size_t len, int is_ortho, size_t batch, int precision) {
+  if (!plan) return CHFFT_ERR_INVALID_ARGUMENT;
+  *plan = nullptr;
+  if (check_prec(precision)) return CHFFT_ERR_INVALID_ARGUMENT;
+  if (dct_type != CHFFT_DCT_II && dct_type != CHFFT_DCT_III) return CHFFT_ERR_INVALID_ARGUMENT;
+  if (len == 0 || (len & 1)) return CHFFT_ERR_INVALID_LENGTH;  // dct1d.rs:54-56
+  if (batch == 0 || batch * len > (size_t)1 << 31) return CHFFT_ERR_INVALID_ARGUMENT;
+  int rc = ensure_runtime();
+  if (rc) return rc;
+  const bool f64 = precision == CHFFT_F64;
+  const size_t h = len / 2;
+  if (!fusable(h, f64)) return CHFFT_ERR_UNSUPPORTED;
+  chfft_plan_s* p = new chfft_plan_s();
+  p->kind = K_DCT;
+  p->f64 = f64;
+  p->n = len;
+  p->batch = batch;
+  p->dct_type = dct_type;
+  p->ortho = is_ortho != 0;
+  if ((rc = build_inner(h, f64, p->inner))) {
+    delete p;
+    return rc;
+  }
+  std::vector<ld_t> co(2 * h), om(2 * (h + 1));
+  for (size_t i = 0; i < h; i++) {
+    ld_t c, s;
+    unit_root(i, len, &c, &s);
+    if (dct_type == CHFFT_DCT_II) {  // (1 + i W_N^i) / 2, dct1d.rs:208-215
+      co[2 * i] = (1.0L - s) / 2.0L;
+      co[2 * i + 1] = c / 2.0L;
+    } else {  // (1 - i W_N^-i) / 2, dct1d.rs:330-338
+      co[2 * i] = (1.0L - s) / 2.0L;
+      co[2 * i + 1] = -c / 2.0L;
+    }
+  }
+  for (size_t k = 0; k <= h; k++) {
+    ld_t c, s;
+    unit_root(k, 4 * len, &c, &s);  // exp(-i pi k / 2N)
+    om[2 * k] = c;
+    om[2 * k + 1] = dct_type == CHFFT_DCT_II ? -s : s;  // DCT-II uses exp(+i pi k / 2N)
+  }
+  if ((rc = upload(co, f64, p->tab_a)) || (rc = upload(om, f64, p->tab_b))) {
+    delete p;
+    return rc;
+  }
+  const bool two = dct_type == CHFFT_DCT_II;
+  Launch f = fused_op(p, two ? LD_DCT2 : LD_DCT3, two ? ST_DCT2 : ST_DCT3, !two, (long long)len, (long long)len,
+                      (int)batch);
+  f.P.N = (long long)len;
+  f.P.tab_a = p->tab_a.p;
+  f.P.tab_b = p->tab_b.p;
+  // dct1d.rs:51-55 and 174-178
+  f.P.ortho = two ? (p->ortho ? std::sqrt(0.5) : 1.0) : (p->ortho ? std::sqrt(0.5) : 0.5);
+  p->fwd.push_back(f);
+  return finish_plan(p, plan);
+}
+
+int chfft_plan_mdct_1d(chfft_plan* plan, size_t len, const double* window, size_t batch, size_t frame_stride,
+                       int precision) {
+  if (!plan) return CHFFT_ERR_INVALID_ARGUMENT;
+  *plan = nullptr;
+  if (check_prec(precision)) return CHFFT_ERR_INVALID_ARGUMENT;
+  if (len == 0 || (len & 3)) return CHFFT_ERR_INVALID_LENGTH;  // mdct1d.rs:102-104
+  if (!window) return CHFFT_ERR_INVALID_ARGUMENT;
+  if (batch == 0 || batch * len > (size_t)1 << 31) return CHFFT_ERR_INVALID_ARGUMENT;
+  int rc = ensure_runtime();
+  if (rc) return rc;
+  const bool f64 = precision == CHFFT_F64;
+  const size_t q = len / 4;
+  if (!fusable(q, f64)) return CHFFT_ERR_UNSUPPORTED;
+  chfft_plan_s* p = new chfft_plan_s();
+  p->kind = K_MDCT;
+  p->f64 = f64;
+  p->n = len;
+  p->batch = batch;
+  p->frame_stride = frame_stride ? frame_stride : len;
+  if ((rc = build_inner(q, f64, p->inner))) {
+    delete p;
+    return rc;
+  }
+  // tw_i = exp(-2 pi i (8i + 1) / 8N) * (2/N)^(1/4) (mdct1d.rs:78-89)
+  std::vector<ld_t> tw(2 * q), w(len);
+  const ld_t sc = sqrtl(sqrtl(2.0L / (ld_t)len));
+  for (size_t i = 0; i < q; i++) {
+    ld_t c, s;
+    unit_root(8 * i + 1, 8 * len, &c, &s);
+    tw[2 * i] = c * sc;
+    tw[2 * i + 1] = s * sc;
+  }
+  for (size_t i = 0; i < len; i++) w[i] = (ld_t)window[i];
+  if ((rc = upload(tw, f64, p->tab_a)) || (rc = upload(w, f64, p->tab_b))) {
+    delete p;
+    return rc;
+  }
+  Launch f = fused_op(p, LD_MDCT, ST_MDCT, false, (long long)p->frame_stride, (long long)(len / 2), (int)batch);
+  f.P.N = (long long)len;
+  f.P.tab_a = p->tab_a.p;
+  f.P.tab_b = p->tab_b.p;
+  p->fwd.push_back(f);
+  Launch b = fused_op(p, LD_IMDCT, ST_IMDCT, false, (long long)(len / 2), (long long)len, (int)batch);
+  b.P.N = (long long)len;
+  b.P.tab_a = p->tab_a.p;
+  b.P.tab_b = p->tab_b.p;
+  p->bwd.push_back(b);
+  return finish_plan(p, plan);
+}
+
+int chfft_plan_destroy(chfft_plan plan) {
+  delete plan;
+  return CHFFT_OK;
+}
+
+}  // extern "C"
+
+namespace chfft {
+namespace {
+
+// Method -> (inverse?, scale), per transform type (see include/chfft_hip.h).
+int method_scale(const chfft_plan_s* p, int method, bool* inverse, double* scale) {
+  const double N = p->kind == K_C2C2D ? (double)p->m * (double)p->n : (double)p->n;
+  switch (p->kind) {
+    case K_C2C:
+    case K_R2C:
+    case K_C2C2D:
+      switch (method) {
+        case CHFFT_FORWARD: case CHFFT_FORWARD0: *inverse = false; *scale = 1.0; return CHFFT_OK;
+        case CHFFT_FORWARDU: *inverse = false; *scale = 1.0 / std::sqrt(N); return CHFFT_OK;
+        case CHFFT_FORWARDN: *inverse = false; *scale = 1.0 / N; return CHFFT_OK;
+        case CHFFT_BACKWARD: *inverse = true; *scale = 1.0 / N; return CHFFT_OK;
+        case CHFFT_BACKWARD0: *inverse = true; *scale = 1.0; return CHFFT_OK;
+        case CHFFT_BACKWARDU: *inverse = true; *scale = 1.0 / std::sqrt(N); return CHFFT_OK;
+        case CHFFT_BACKWARDN:
+          if (p->kind == K_C2C2D) return CHFFT_ERR_UNSUPPORTED;  // CFft2D has no backwardn
+          *inverse = true;
+          *scale = 1.0 / N;
+          return CHFFT_OK;
+      }
+      return CHFFT_ERR_INVALID_ARGUMENT;
+    case K_DCT: {
+      const double u = std::sqrt(2.0 / N);  // dct1d.rs:218
+      *inverse = false;
+      switch (method) {
+        case CHFFT_FORWARD: case CHFFT_FORWARD0: *scale = 1.0; return CHFFT_OK;
+        case CHFFT_FORWARDU: *scale = u; return CHFFT_OK;
+        case CHFFT_FORWARDN: *scale = u * u; return CHFFT_OK;
+      }
+      return method >= 0 && method <= 7 ? CHFFT_ERR_UNSUPPORTED : CHFFT_ERR_INVALID_ARGUMENT;
+    }
+    case K_MDCT: {
+      const double h = (double)(p->n >> 1);
+      switch (method) {
+        case CHFFT_FORWARD: *inverse = false; *scale = std::sqrt(h); return CHFFT_OK;
+        case CHFFT_FORWARDU: *inverse = false; *scale = 1.0; return CHFFT_OK;
+        case CHFFT_BACKWARD: *inverse = true; *scale = 1.0 / std::sqrt(h); return CHFFT_OK;
+        case CHFFT_BACKWARDU: *inverse = true; *scale = 1.0; return CHFFT_OK;
+      }
+      return method >= 0 && method <= 7 ? CHFFT_ERR_UNSUPPORTED : CHFFT_ERR_INVALID_ARGUMENT;
+    }
+  }
+  return CHFFT_ERR_INVALID_ARGUMENT;
+}
+
+void io_bytes(const chfft_plan_s* p, bool inverse, size_t* ib, size_t* ob) {
+  const size_t e = p->f64 ? 8 : 4;  // bytes per real
+  const size_t B = p->batch, n = p->n;
+  switch (p->kind) {
+    case K_C2C: *ib = *ob = 2 * e * n * B; return;
+    case K_C2C2D: *ib = *ob = 2 * e * n * p->m * B; return;
+    case K_R2C:
+      if (!inverse) {
+        *ib = e * n * B;
+        *ob = 2 * e * (n / 2 + 1) * B;
+      } else {
+        *ib = 2 * e * (n / 2 + 1) * B;
+        *ob = e * n * B;
+      }
+      return;
+    case K_DCT: *ib = *ob = e * n * B; return;
+    case K_MDCT:
+      if (!inverse) {
+        *ib = e * ((B - 1) * p->frame_stride + n);
+        *ob = e * (n / 2) * B;
+      } else {
+        *ib = e * (n / 2) * B;
+        *ob = e * n * B;
+      }
+      return;
+  }
+}
+
+}  // namespace
+}  // namespace chfft
+
+extern "C" {
+
+int chfft_exec(chfft_plan plan, const void* in, void* out, int method, void* stream) {
+  if (!plan || !in || !out) return CHFFT_ERR_INVALID_ARGUMENT;
+  bool inverse;
+  double scale;
+  int rc = method_scale(plan, method, &inverse, &scale);
+  if (rc) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const std::vector<Launch>& L = inverse ? plan->bwd : plan->fwd;
+  for (const Launch& l0 : L) {
+    const void* slots_src[3] = {in, out, plan->scratch.p};
+    void* slots_dst[3] = {const_cast<void*>(in), out, plan->scratch.p};
+    if (l0.ld < 0) {  // length-1 copy
+      size_t ib, ob;
+      io_bytes(plan, inverse, &ib, &ob);
+      if (slots_src[l0.src] != slots_dst[l0.dst] &&
+          launch_copy(slots_src[l0.src], slots_dst[l0.dst], ib, st) != hipSuccess)
+        return CHFFT_ERR_HIP;
+      continue;
+    }
+    KParams P = l0.P;
+    P.src = slots_src[l0.src];
+    P.dst = slots_dst[l0.dst];
+    P.scale = l0.scaled ? scale * l0.scale_mul : 1.0;
+    if (launch_fused(plan->f64, l0.ld, l0.st, P, l0.grid, l0.block, l0.lds, st) != hipSuccess)
+      return CHFFT_ERR_HIP;
+  }
+  return CHFFT_OK;
+}
+
+int chfft_plan_io_bytes(chfft_plan plan, int method, size_t* in_bytes, size_t* out_bytes) {
+  if (!plan || !in_bytes || !out_bytes) return CHFFT_ERR_INVALID_ARGUMENT;
+  bool inverse;
+  double scale;
+  int rc = method_scale(plan, method, &inverse, &scale);
+  if (rc) return rc;
+  io_bytes(plan, inverse, in_bytes, out_bytes);
+  return CHFFT_OK;
+}
+
+size_t chfft_plan_scratch_bytes(chfft_plan plan) { return plan ? plan->scratch.bytes : 0; }
+
+int chfft_exec_host(chfft_plan plan, const void* in, void* out, int method) {
+  if (!plan || !in || !out) return CHFFT_ERR_INVALID_ARGUMENT;
+  size_t ib, ob;
+  int rc = chfft_plan_io_bytes(plan, method, &ib, &ob);
+  if (rc) return rc;
+  if ((rc = dev_alloc(plan->stage_in, ib)) || (rc = dev_alloc(plan->stage_out, ob))) return rc;
+  if (hipMemcpy(plan->stage_in.p, in, ib, hipMemcpyHostToDevice) != hipSuccess) return CHFFT_ERR_HIP;
+  if ((rc = chfft_exec(plan, plan->stage_in.p, plan->stage_out.p, method, nullptr))) return rc;
+  if (hipMemcpy(out, plan->stage_out.p, ob, hipMemcpyDeviceToHost) != hipSuccess) return CHFFT_ERR_HIP;
+  return CHFFT_OK;
+}
+
+int chfft_plan_describe(chfft_plan plan, char* buf, size_t buf_len) {
+  if (!plan) return -1;
+  std::string s;
+  for (int dir = 0; dir < 2; dir++) {
+    const std::vector<Launch>& L = dir ? plan->bwd : plan->fwd;
+    if (L.empty()) continue;
+    s += dir ? "inverse:\n" : "forward:\n";
+    for (const Launch& l : L) {
+      char line[512];
+      snprintf(line, sizeof line, "  %s | G=%d tpb=%d block=%u grid=%u lds=%zu\n", l.what.c_str(), l.P.G, l.P.tpb,
+               l.block.x, l.grid.x, l.lds);
+      s += line;
+    }
+  }
+  if (buf && buf_len) {
+    size_t k = std::min(buf_len - 1, s.size());
+    memcpy(buf, s.data(), k);
+    buf[k] = 0;
+  }
+  return (int)plan->fwd.size();
+}
+
+}  // extern "C"

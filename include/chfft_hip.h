@@ -1,0 +1,120 @@
+/*
+ * chfft_hip.h — C ABI of the MI355X (gfx950) transform engine.
+ *
+ * Drop-in boundary for chfft 0.3.4's five public transform types
+ * (lib.rs:21-25: CFft1D, CFft2D, Dct1D/DctType, Mdct1D, RFft1D). The reference
+ * has no FFI of its own; each entry point below replaces the Rust method it
+ * cites, and a language binding (the Rust shim in INTEGRATION.md, the Python
+ * mirror in chfft_amd/) keeps the reference's method names on top of it.
+ *
+ * Conventions
+ *  - Plans are created for one length (or shape) and a batch count, and own
+ *    their device tables and scratch. A plan is used by one host thread at a
+ *    time (the reference takes &mut self on every method); distinct plans may
+ *    run concurrently.
+ *  - chfft_exec takes DEVICE pointers and is asynchronous on `stream`
+ *    (a hipStream_t; NULL = default stream). chfft_exec_host takes HOST
+ *    pointers, copies through the plan's staging buffers and blocks.
+ *  - Complex data is interleaved {re, im} (num-complex Complex<T> layout),
+ *    batch-major: transform b starts at element b * (its length).
+ *  - precision: CHFFT_F32 (FP32 butterflies, FP64-generated tables rounded
+ *    once) or CHFFT_F64 (the plain method).
+ *  - in == out is allowed for every complex-to-complex plan (the reference's
+ *    in-place *i methods, cfft1d.rs:403-455).
+ */
+#ifndef CHFFT_HIP_H
+#define CHFFT_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chfft_plan_s* chfft_plan;
+
+typedef enum {
+  CHFFT_OK = 0,
+  /* construct/setup/exec length errors: the reference panics with
+     "invalid length" (rfft1d.rs:74-76,116-122; dct1d.rs:54-56,237-243;
+     mdct1d.rs:102-104,198-204) */
+  CHFFT_ERR_INVALID_LENGTH = 1,
+  CHFFT_ERR_INVALID_ARGUMENT = 2, /* null pointer, unknown enum value */
+  CHFFT_ERR_UNSUPPORTED = 3,      /* method the transform type does not have */
+  CHFFT_ERR_HIP = 4,              /* a HIP runtime call failed */
+  CHFFT_ERR_OUT_OF_MEMORY = 5
+} chfft_status;
+
+typedef enum { CHFFT_F32 = 0, CHFFT_F64 = 1 } chfft_precision;
+
+/* Methods, named and scaled as the reference's (cfft1d.rs:270-389,
+   rfft1d.rs:199-312, cfft2d.rs:150-324, dct1d.rs:98-144, mdct1d.rs:149-195).
+   The in-place *i variants are these methods with in == out. */
+typedef enum {
+  CHFFT_FORWARD = 0,   /* scale 1                (MDCT: sqrt(N/2)) */
+  CHFFT_FORWARD0 = 1,  /* scale 1 */
+  CHFFT_FORWARDU = 2,  /* 1/sqrt(N)              (DCT: sqrt(2/N); MDCT: 1) */
+  CHFFT_FORWARDN = 3,  /* 1/N                    (DCT: 2/N) */
+  CHFFT_BACKWARD = 4,  /* 1/N                    (MDCT: 1/sqrt(N/2)) */
+  CHFFT_BACKWARD0 = 5, /* 1 */
+  CHFFT_BACKWARDU = 6, /* 1/sqrt(N)              (MDCT: 1) */
+  CHFFT_BACKWARDN = 7  /* 1/N */
+} chfft_method;
+
+typedef enum { CHFFT_DCT_II = 2, CHFFT_DCT_III = 3 } chfft_dct_type;
+
+/* CFft1D::with_len / setup (cfft1d.rs:83-92, 103-206): `batch` transforms of
+   `len` complex points. len >= 1; 7-smooth lengths use mixed radix, others
+   Bluestein (MAX_PRIME = 7, cfft1d.rs:104). */
+int chfft_plan_c2c_1d(chfft_plan* plan, size_t len, size_t batch, int precision);
+
+/* RFft1D::new / setup (rfft1d.rs:73-113): len even (else
+   CHFFT_ERR_INVALID_LENGTH). Forward: len reals -> len/2+1 complex per
+   transform; backward: the reverse. */
+int chfft_plan_r2c_1d(chfft_plan* plan, size_t len, size_t batch, int precision);
+
+/* CFft2D::with_len / setup (cfft2d.rs:93-124): len_m rows of len_n complex
+   points, row-major, `batch` matrices. */
+int chfft_plan_c2c_2d(chfft_plan* plan, size_t len_m, size_t len_n, size_t batch, int precision);
+
+/* Dct1D::new / setup (dct1d.rs:53-86): dct_type CHFFT_DCT_II or
+   CHFFT_DCT_III, len even, is_ortho as the reference's flag. len reals in,
+   len reals out. */
+int chfft_plan_dct_1d(chfft_plan* plan, int dct_type, size_t len, int is_ortho, size_t batch, int precision);
+
+/* Mdct1D::new / with_sine / with_vorbis / setup (mdct1d.rs:55-137): len % 4
+   == 0. `window` holds the len window values w(len, p) (the reference
+   evaluates its window closure into the same table, mdct1d.rs:91-93).
+   Forward reads `batch` frames of len samples, frame b starting at sample
+   b * frame_stride (frame_stride = len for independent frames, len/2 for a
+   50 % overlapped signal; 0 means len), and writes len/2 coefficients per
+   frame. Backward reads len/2 coefficients and writes len windowed samples
+   per frame. */
+int chfft_plan_mdct_1d(chfft_plan* plan, size_t len, const double* window, size_t batch, size_t frame_stride,
+                       int precision);
+
+int chfft_plan_destroy(chfft_plan plan);
+
+/* Run `method` on device buffers, asynchronously on `stream`. */
+int chfft_exec(chfft_plan plan, const void* in, void* out, int method, void* stream);
+
+/* Run `method` on host buffers; blocks until the result is in `out`. */
+int chfft_exec_host(chfft_plan plan, const void* in, void* out, int method);
+
+/* Sizes in bytes of one exec's input and output for `method`, and of the
+   plan's device scratch. */
+int chfft_plan_io_bytes(chfft_plan plan, int method, size_t* in_bytes, size_t* out_bytes);
+size_t chfft_plan_scratch_bytes(chfft_plan plan);
+
+/* Human-readable kernel schedule of the plan (launch count, per launch the
+   operators, FFT length, radix passes, transforms per workgroup, workgroup
+   size). Returns the number of kernel launches one exec makes. */
+int chfft_plan_describe(chfft_plan plan, char* buf, size_t buf_len);
+
+const char* chfft_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHFFT_HIP_H */

@@ -1,0 +1,248 @@
+"""GPU parity: the HIP engine (through the C ABI, via the Python mirror of the
+reference API) against the CPU oracle on the same seeded inputs.
+
+Bars: FP64 plain method rel-L2 <= 1e-12 against the FP64 oracle (the
+restatement of chfft f64); FP32 form rel-L2 <= 1e-6 * log2(N) against the FP64
+oracle evaluated on the same FP32-rounded inputs. Length sweeps follow the
+reference tests (cfft1d.rs:566-603, rfft1d.rs:390-402, dct1d.rs:483-525,
+mdct1d.rs:384-466, cfft2d.rs:454-488); config sizes follow the benchmark
+workloads at reduced batch.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from chfft_amd import CFft1D, CFft2D, Dct1D, DctType, Mdct1D, RFft1D
+from tests.gpu_util import TOL_F64, crand, rel_l2, rng, tol_f32
+
+pytestmark = pytest.mark.gpu
+
+C2C_METHODS = ["forward", "forward0", "forwardu", "forwardn", "backward", "backward0", "backwardu", "backwardn"]
+PRIMES = [2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97]
+
+
+def check(gpu, ref, f64, n):
+    e = rel_l2(gpu, ref)
+    tol = TOL_F64 if f64 else tol_f32(n)
+    assert e <= tol, f"rel-L2 {e:.3e} > {tol:.1e} (n={n})"
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_c2c_sweep_all_methods(dtype):
+    # cfft1d.rs:566-592: N = 1..99, every exec variant, 10 vectors per length
+    f64 = dtype == np.float64
+    cdt = np.complex128 if f64 else np.complex64
+    r = rng(1)
+    fft = CFft1D.new(dtype)
+    for n in range(1, 100):
+        x = crand(r, (10, n), cdt)
+        for m in C2C_METHODS:
+            got = getattr(fft, m)(x)
+            ref = oracle.c2c(x.astype(np.complex128), m)
+            check(got, ref, f64, n)
+        # in-place variants (cfft1d.rs:526-542)
+        for m, mi in [("forward0", "forward0i"), ("backward0", "backward0i"), ("forwardu", "forwardui"),
+                      ("backwardu", "backwardui")]:
+            y = x.copy()
+            getattr(fft, mi)(y)
+            check(y, oracle.c2c(x.astype(np.complex128), m), f64, n)
+
+
+def test_c2c_primes_one_plan():
+    # cfft1d.rs:594-603: one plan object reused across prime lengths (replans)
+    r = rng(2)
+    fft = CFft1D.new(np.float64)
+    for n in PRIMES:
+        x = crand(r, (10, n))
+        got = fft.forward(x)
+        check(got, oracle.c2c(x, "forward"), True, n)
+        back = fft.backward(got)
+        check(back, x, True, n)
+
+
+def test_readme_known_answer():
+    # README.md:33-37 / cfft1d.rs:43-48: [2, 1+i, 3i, 2+4i]
+    x = np.array([2, 1 + 1j, 3j, 2 + 4j])
+    got = CFft1D.with_len(4).forward(x)
+    np.testing.assert_allclose(got, [5 + 8j, -1 - 2j, -1 - 2j, 5 - 4j], atol=1e-14)
+
+
+@pytest.mark.parametrize("n,batch", [(4096, 256), (43200, 4), (4093, 8), (10007, 4), (65537, 2),
+                                     (8192, 16), (16384, 4), (20000, 2), (1 << 20, 1), (3 ** 9, 2),
+                                     (7 * 11 * 13 * 17, 2), (1048573, 1)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_c2c_config_sizes(n, batch, dtype):
+    f64 = dtype == np.float64
+    cdt = np.complex128 if f64 else np.complex64
+    r = rng(n)
+    x = (r.standard_normal((batch, n)) + 1j * r.standard_normal((batch, n))).astype(cdt)
+    fft = CFft1D.with_len(n, dtype)
+    y = fft.forward(x)
+    xr = x.astype(np.complex128)
+    check(y, np.fft.fft(xr) if n > 70000 else oracle.c2c(xr, "forward"), f64, n)
+    z = fft.backward(y)
+    check(z, xr, f64, n)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_rfft_sweep(dtype):
+    # rfft1d.rs:390-402: even N = 2..198, all scalings, round trips
+    f64 = dtype == np.float64
+    r = rng(3)
+    for n in range(2, 200, 2):
+        x = r.random((10, n)).astype(dtype)
+        f = RFft1D.new(n, dtype)
+        for m in ["forward", "forward0", "forwardu", "forwardn"]:
+            got = f.__getattribute__(m)(x)
+            check(got, oracle.rfft(x.astype(np.float64), m), f64, n)
+        X = oracle.rfft(x.astype(np.float64), "forward")
+        for m in ["backward", "backward0", "backwardu", "backwardn"]:
+            got = getattr(f, m)(X.astype(np.complex128 if f64 else np.complex64))
+            check(got, oracle.rfft(X, m, n=n), f64, n)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_rfft_config_size(dtype):
+    f64 = dtype == np.float64
+    r = rng(4)
+    n = 8192
+    x = r.standard_normal((64, n)).astype(dtype)
+    f = RFft1D.new(n, dtype)
+    X = f.forward(x)
+    check(X, oracle.rfft(x.astype(np.float64), "forward"), f64, n)
+    xb = f.backward(X)
+    check(xb, x.astype(np.float64), f64, n)
+
+
+def test_rfft_invalid_length():
+    # rfft1d.rs:404-422
+    with pytest.raises(ValueError, match="invalid length"):
+        RFft1D.new(11)
+    f = RFft1D.new(4)
+    with pytest.raises(ValueError, match="invalid length"):
+        f.forward(np.arange(5.0))
+    with pytest.raises(ValueError, match="invalid length"):
+        f.backward(np.arange(5.0).astype(np.complex128))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_dct_sweep(dtype):
+    # dct1d.rs:436-503: DCT-II forwardu/forwardn/forward0 against DCT-III pairs
+    f64 = dtype == np.float64
+    r = rng(5)
+    for n in range(2, 200, 2):
+        x = r.random((10, n)).astype(dtype)
+        d2 = Dct1D.new(DctType.Two, n, False, dtype)
+        d3 = Dct1D.new(DctType.Three, n, False, dtype)
+        for m in ["forward", "forward0", "forwardu", "forwardn"]:
+            check(getattr(d2, m)(x), oracle.dct(x.astype(np.float64), 2, False, m), f64, n)
+            check(getattr(d3, m)(x), oracle.dct(x.astype(np.float64), 3, False, m), f64, n)
+        check(d3.forwardu(d2.forwardu(x)), x, f64, n)
+        check(d3.forward0(d2.forwardn(x)), x, f64, n)
+
+
+def test_dct_ortho_and_setup():
+    # is_ortho scales input element 0 (dct1d.rs:220-224, 343-347); setup paths
+    # dct1d.rs:505-525
+    r = rng(6)
+    for n in [2, 4, 6, 10, 64, 126]:
+        x = r.random((4, n))
+        for t in (2, 3):
+            d = Dct1D.new(DctType(t), n, True)
+            check(d.forward(x), oracle.dct(x, t, True, "forward"), True, n)
+        d = Dct1D.new(DctType.Two, 2 * n, True)
+        d.setup(DctType.Three, n, False)
+        check(d.forwardn(x), oracle.dct(x, 3, False, "forwardn"), True, n)
+
+
+def test_dct_config_size():
+    r = rng(7)
+    n = 1024
+    x = (2 * r.random((256, n)) - 1)
+    for dtype in (np.float64, np.float32):
+        xs = x.astype(dtype)
+        for t in (2, 3):
+            got = Dct1D.new(DctType(t), n, False, dtype).forward(xs)
+            check(got, oracle.dct(xs.astype(np.float64), t, False, "forward"), dtype == np.float64, n)
+
+
+def test_dct_invalid_length():
+    with pytest.raises(ValueError, match="invalid length"):
+        Dct1D.new(DctType.Two, 11, False)
+    d = Dct1D.new(DctType.Two, 8, False)
+    with pytest.raises(ValueError, match="invalid length"):
+        d.forward(np.arange(10.0))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("kind", ["sine", "vorbis"])
+def test_mdct_sweep(dtype, kind):
+    # mdct1d.rs:384-448: N = 4..396 step 4, forward/backward and unitary pairs
+    f64 = dtype == np.float64
+    r = rng(8)
+    for n in range(4, 400, 4):
+        x = r.random((10, n)).astype(dtype)
+        md = Mdct1D.with_sine(n, dtype) if kind == "sine" else Mdct1D.with_vorbis(n, dtype)
+        w = oracle.window(n, kind)
+        for m in ["forward", "forwardu"]:
+            check(getattr(md, m)(x), oracle.mdct(x.astype(np.float64), w, m), f64, n)
+        c = oracle.mdct(x.astype(np.float64), w, "forward")
+        for m in ["backward", "backwardu"]:
+            check(getattr(md, m)(c.astype(dtype)), oracle.mdct(c, w, m), f64, n)
+
+
+def test_mdct_setup_and_invalid():
+    # mdct1d.rs:450-486
+    md = Mdct1D.with_sine(64)
+    md.setup(16)
+    r = rng(9)
+    x = r.random((3, 16))
+    check(md.forward(x), oracle.mdct(x, oracle.window(16), "forward"), True, 16)
+    with pytest.raises(ValueError, match="invalid length"):
+        Mdct1D.with_sine(10)
+    md = Mdct1D.with_sine(8)
+    with pytest.raises(ValueError, match="invalid length"):
+        md.forward(np.arange(10.0))
+    with pytest.raises(ValueError, match="invalid length"):
+        md.backward(np.arange(8.0))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_mdct_overlapped_signal(dtype):
+    # the MDCT workload: N = 2048, hop 1024 over a signal (reduced length)
+    n, hop = 2048, 1024
+    r = rng(10)
+    sig = r.standard_normal(hop * 64 + n).astype(dtype)
+    md = Mdct1D.with_sine(n, dtype)
+    got = md.forward_signal(sig, hop)
+    frames = (len(sig) - n) // hop + 1
+    ref = oracle.mdct(sig.astype(np.float64), oracle.window(n), "forward", n=n, hop=hop, frames=frames)
+    check(got, ref, dtype == np.float64, n)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_cfft2d_sweep(dtype):
+    # cfft2d.rs:454-488: 1..9 x 1..9, forward then backward
+    f64 = dtype == np.float64
+    cdt = np.complex128 if f64 else np.complex64
+    r = rng(11)
+    fft = CFft2D.new(dtype)
+    for m in range(1, 10):
+        for n in range(1, 10):
+            x = crand(r, (m, n), cdt)
+            for meth in ["forward", "forward0", "forwardu", "forwardn", "backward", "backward0", "backwardu"]:
+                got = getattr(fft, meth)(x)
+                check(got, oracle.c2c2d(x.astype(np.complex128), meth), f64, m * n)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_cfft2d_config_size(dtype):
+    f64 = dtype == np.float64
+    cdt = np.complex128 if f64 else np.complex64
+    r = rng(12)
+    m = n = 2048
+    x = (r.standard_normal((m, n)) + 1j * r.standard_normal((m, n))).astype(cdt)
+    f = CFft2D.with_len(m, n, dtype)
+    y = f.forward(x)
+    check(y, np.fft.fft2(x.astype(np.complex128)), f64, m * n)
+    check(f.backward(y), x.astype(np.complex128), f64, m * n)
