@@ -52,6 +52,8 @@ def _signatures():
         "chfft_plan_destroy": ([vp], ci),
         "chfft_exec": ([vp, vp, vp, ci, vp], ci),
         "chfft_exec_host": ([vp, vp, vp, ci], ci),
+        "chfft_exec_range": ([vp, vp, vp, ci, ci, ci, vp], ci),
+        "chfft_plan_launch_count": ([vp, ci], ci),
         "chfft_plan_io_bytes": ([vp, ci, ctypes.POINTER(sz), ctypes.POINTER(sz)], ci),
         "chfft_plan_scratch_bytes": ([vp], sz),
         "chfft_plan_describe": ([vp, ctypes.c_char_p, sz], ci),

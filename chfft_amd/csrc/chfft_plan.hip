@@ -943,7 +943,7 @@ void io_bytes(const chfft_plan_s* p, bool inverse, size_t* ib, size_t* ob) {
 
 extern "C" {
 
-int chfft_exec(chfft_plan plan, const void* in, void* out, int method, void* stream) {
+int chfft_exec_range(chfft_plan plan, const void* in, void* out, int method, int first, int count, void* stream) {
   if (!plan || !in || !out) return CHFFT_ERR_INVALID_ARGUMENT;
   bool inverse;
   double scale;
@@ -951,7 +951,9 @@ int chfft_exec(chfft_plan plan, const void* in, void* out, int method, void* str
   if (rc) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const std::vector<Launch>& L = inverse ? plan->bwd : plan->fwd;
-  for (const Launch& l0 : L) {
+  if (first < 0 || count < 0 || (size_t)first + (size_t)count > L.size()) return CHFFT_ERR_INVALID_ARGUMENT;
+  for (int li = first; li < first + count; li++) {
+    const Launch& l0 = L[li];
     const void* slots_src[3] = {in, out, plan->scratch.p};
     void* slots_dst[3] = {const_cast<void*>(in), out, plan->scratch.p};
     if (l0.ld < 0) {  // length-1 copy
@@ -970,6 +972,21 @@ int chfft_exec(chfft_plan plan, const void* in, void* out, int method, void* str
       return CHFFT_ERR_HIP;
   }
   return CHFFT_OK;
+}
+
+int chfft_exec(chfft_plan plan, const void* in, void* out, int method, void* stream) {
+  int n = chfft_plan_launch_count(plan, method);
+  if (n < 0) return -n;
+  return chfft_exec_range(plan, in, out, method, 0, n, stream);
+}
+
+int chfft_plan_launch_count(chfft_plan plan, int method) {
+  if (!plan) return -CHFFT_ERR_INVALID_ARGUMENT;
+  bool inverse;
+  double scale;
+  int rc = method_scale(plan, method, &inverse, &scale);
+  if (rc) return -rc;
+  return (int)(inverse ? plan->bwd.size() : plan->fwd.size());
 }
 
 int chfft_plan_io_bytes(chfft_plan plan, int method, size_t* in_bytes, size_t* out_bytes) {

@@ -98,6 +98,15 @@ int chfft_plan_destroy(chfft_plan plan);
 /* Run `method` on device buffers, asynchronously on `stream`. */
 int chfft_exec(chfft_plan plan, const void* in, void* out, int method, void* stream);
 
+/* Profiling hook (no counterpart in the reference): run only kernel launches
+   [first, first + count) of chfft_exec's schedule for `method`, so that a
+   caller can bracket each kernel with its own events. Running every range in
+   order equals chfft_exec. */
+int chfft_exec_range(chfft_plan plan, const void* in, void* out, int method, int first, int count, void* stream);
+
+/* Number of kernel launches chfft_exec makes for `method`. */
+int chfft_plan_launch_count(chfft_plan plan, int method);
+
 /* Run `method` on host buffers; blocks until the result is in `out`. */
 int chfft_exec_host(chfft_plan plan, const void* in, void* out, int method);
 
