@@ -279,20 +279,21 @@ __device__ __forceinline__ long long io_addr(const IoDesc& d, int g, long long i
 
 // Element n (of Lfft) of transform g for the load stage, before conjugation.
 template <typename T, int LD>
-__device__ __forceinline__ cx<T> load_elem(const KParams& P, int g, int n) {
+__device__ __forceinline__ cx<T> load_elem(const KParams& P, const void* src, int g, int n) {
   const cx<T> zero = mk(T(0), T(0));
   if constexpr (LD == LD_PLAIN) {
-    return reinterpret_cast<const cx<T>*>(P.src)[io_addr(P.in, g, n)];
+    return reinterpret_cast<const cx<T>*>(src)[io_addr(P.in, g, n)];
   } else if constexpr (LD == LD_FS_CHIRP) {
     const int b = g / P.in.ncols;
     const int c = g - b * P.in.ncols;
     const long long s = (long long)c * P.in.cstride + (long long)n * P.in.estride;
     if (s >= P.N) return zero;
-    const cx<T> v = reinterpret_cast<const cx<T>*>(P.src)[(long long)b * P.in.bdist + s];
+    cx<T> v = reinterpret_cast<const cx<T>*>(src)[(long long)b * P.in.bdist + s];
+    if (P.conj_in) v = cj(v);  // inverse: conjugate the input, not the chirp
     return v * reinterpret_cast<const cx<T>*>(P.chirp)[s];
   } else if constexpr (LD == LD_R2C_BACK) {
     // rfft1d.rs:161-177 (complex input, N/2+1 bins per row)
-    const cx<T>* X = reinterpret_cast<const cx<T>*>(P.src) + (long long)g * P.in.bdist;
+    const cx<T>* X = reinterpret_cast<const cx<T>*>(src) + (long long)g * P.in.bdist;
     const cx<T>* bc = reinterpret_cast<const cx<T>*>(P.tab_a);
     const int N = (int)P.N, h = N >> 1, q = (N + 3) >> 2;
     if (n == 0) {
@@ -306,7 +307,7 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, int g, int n) {
     return (n < q) ? xi - x : xh + cj(x);
   } else if constexpr (LD == LD_DCT2) {
     // dct1d.rs:245-263 (real input)
-    const T* x = reinterpret_cast<const T*>(P.src) + (long long)g * P.in.bdist;
+    const T* x = reinterpret_cast<const T*>(src) + (long long)g * P.in.bdist;
     const int N = (int)P.N, h = N >> 1, q = h >> 1;
     const T o = (T)P.ortho;
     if (q == 0) return mk(x[0] * o, x[1]);
@@ -317,7 +318,7 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, int g, int n) {
     return mk(x[4 * i + 3], x[4 * i + 1]);
   } else if constexpr (LD == LD_DCT3) {
     // dct1d.rs:367-391 (real input)
-    const T* x = reinterpret_cast<const T*>(P.src) + (long long)g * P.in.bdist;
+    const T* x = reinterpret_cast<const T*>(src) + (long long)g * P.in.bdist;
     const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
     const cx<T>* om = reinterpret_cast<const cx<T>*>(P.tab_b);
     const int N = (int)P.N, h = N >> 1, q = (N + 3) >> 2;
@@ -334,7 +335,7 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, int g, int n) {
     return (n < q) ? j - xx : cj(k) + cj(xx);
   } else if constexpr (LD == LD_MDCT) {
     // mdct1d.rs:206-230 (frames `bdist` samples apart)
-    const T* x = reinterpret_cast<const T*>(P.src) + (long long)g * P.in.bdist;
+    const T* x = reinterpret_cast<const T*>(src) + (long long)g * P.in.bdist;
     const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
     const T* w = reinterpret_cast<const T*>(P.tab_b);
     const int N = (int)P.N, q = N >> 2, h = N >> 1, q3 = h + q, q5 = h + q3, lo = q - (q >> 1);
@@ -346,7 +347,7 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, int g, int n) {
       v = mk(x[q3 - 1 - m] * w[q3 - 1 - m] - x[m - q] * w[m - q], x[q + m] * w[q + m] + x[q5 - 1 - m] * w[q5 - 1 - m]);
     return v * tw[n];
   } else {  // LD_IMDCT, mdct1d.rs:261-265
-    const T* c = reinterpret_cast<const T*>(P.src) + (long long)g * P.in.bdist;
+    const T* c = reinterpret_cast<const T*>(src) + (long long)g * P.in.bdist;
     const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
     const int h = (int)(P.N >> 1);
     return mk(c[2 * n], c[h - 1 - 2 * n]) * scl(tw[n], T(-2));
@@ -354,7 +355,7 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, int g, int n) {
 }
 
 template <typename T, int LD>
-__device__ void load_stage(const KParams& P, cx<T>* lds, int g0, int nt) {
+__device__ void load_stage(const KParams& P, const void* src, cx<T>* lds, int g0, int nt) {
   const int Lf = P.Lfft;
   const int total = P.tpb * Lf;
   const cx<T>* chirp = reinterpret_cast<const cx<T>*>(P.chirp);
@@ -369,8 +370,10 @@ __device__ void load_stage(const KParams& P, cx<T>* lds, int g0, int nt) {
     }
     cx<T> v = mk(T(0), T(0));
     if (t < nt && n < P.L) {  // n >= L only under a fused Bluestein: zero padding
-      v = load_elem<T, LD>(P, g0 + t, n);
-      if (P.conj_in) v = cj(v);
+      v = load_elem<T, LD>(P, src, g0 + t, n);
+      if constexpr (LD != LD_FS_CHIRP) {
+        if (P.conj_in) v = cj(v);
+      }
       if (P.has_chirp && n < P.L) v = v * chirp[n];
     }
     lds[t * P.Lp + n] = v;
@@ -418,12 +421,12 @@ __device__ __forceinline__ cx<T> getz(const KParams& P, const cx<T>* lds, int t,
 }
 
 template <typename T, int ST>
-__device__ void store_stage(const KParams& P, const cx<T>* lds, int g0, int nt) {
+__device__ void store_stage(const KParams& P, void* dst, const cx<T>* lds, int g0, int nt) {
   const T s = (T)P.scale;
   if constexpr (ST == ST_PLAIN || ST == ST_TW || ST == ST_FS_DECHIRP) {
     const int L = P.L;
     const int total = P.tpb * L;
-    cx<T>* out = reinterpret_cast<cx<T>*>(P.dst);
+    cx<T>* out = reinterpret_cast<cx<T>*>(dst);
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
       int t, k;
       if (P.colmajor_st) {
@@ -466,7 +469,7 @@ __device__ void store_stage(const KParams& P, const cx<T>* lds, int g0, int nt) 
       const int t = e / per;
       const int r = e - t * per;
       if (t >= nt) continue;
-      cx<T>* out = reinterpret_cast<cx<T>*>(P.dst) + (long long)(g0 + t) * P.out.bdist;
+      cx<T>* out = reinterpret_cast<cx<T>*>(dst) + (long long)(g0 + t) * P.out.bdist;
       cx<T> v;
       if (r == 0 || r == h) {
         const cx<T> z = getz(P, lds, t, 0);
@@ -492,7 +495,7 @@ __device__ void store_stage(const KParams& P, const cx<T>* lds, int g0, int nt) 
       const int t = e / per;
       const int i = e - t * per;
       if (t >= nt) continue;
-      T* ret = reinterpret_cast<T*>(P.dst) + (long long)(g0 + t) * P.out.bdist;
+      T* ret = reinterpret_cast<T*>(dst) + (long long)(g0 + t) * P.out.bdist;
       if (i == 0) {
         const cx<T> z = getz(P, lds, t, 0);
         ret[0] = (z.re + z.im) * s;
@@ -522,7 +525,7 @@ __device__ void store_stage(const KParams& P, const cx<T>* lds, int g0, int nt) 
       const int t = e / per;
       const int i = e - t * per;
       if (t >= nt) continue;
-      T* ret = reinterpret_cast<T*>(P.dst) + (long long)(g0 + t) * P.out.bdist;
+      T* ret = reinterpret_cast<T*>(dst) + (long long)(g0 + t) * P.out.bdist;
       const cx<T> a = getz(P, lds, t, i);
       if (i < q2) {
         const cx<T> b = getz(P, lds, t, h - 1 - i);
@@ -544,7 +547,7 @@ __device__ void store_stage(const KParams& P, const cx<T>* lds, int g0, int nt) 
       const int t = e / q;
       const int i = e - t * q;
       if (t >= nt) continue;
-      T* ret = reinterpret_cast<T*>(P.dst) + (long long)(g0 + t) * P.out.bdist;
+      T* ret = reinterpret_cast<T*>(dst) + (long long)(g0 + t) * P.out.bdist;
       const cx<T> v = getz(P, lds, t, i) * scl(tw[i], s);
       ret[2 * i] = -v.re;
       ret[h - 1 - 2 * i] = v.im;
@@ -558,7 +561,7 @@ __device__ void store_stage(const KParams& P, const cx<T>* lds, int g0, int nt) 
       const int t = e / q;
       const int i = e - t * q;
       if (t >= nt) continue;
-      T* ret = reinterpret_cast<T*>(P.dst) + (long long)(g0 + t) * P.out.bdist;
+      T* ret = reinterpret_cast<T*>(dst) + (long long)(g0 + t) * P.out.bdist;
       const cx<T> v = getz(P, lds, t, i) * scl(tw[i], s);
       const int m = 2 * i;
       ret[q3 - 1 - m] = v.re * w[q3 - 1 - m];
@@ -574,6 +577,21 @@ __device__ void store_stage(const KParams& P, const cx<T>* lds, int g0, int nt) 
   }
 }
 
+// One LDS-resident batch of tpb transforms: load, passes, [Bluestein middle],
+// store. The store stage is not followed by a barrier.
+template <typename T, int LD, int ST>
+__device__ __forceinline__ void block_transform(const KParams& P, const void* src, void* dst, cx<T>* lds, int g0,
+                                                int nt) {
+  load_stage<T, LD>(P, src, lds, g0, nt);
+  run_passes<T>(lds, P.seq, P.Lp, P.tpb);
+  if (P.has_mid) {
+    mid_stage<T>(P, lds, g0);
+    run_passes<T>(lds, P.seq, P.Lp, P.tpb);
+    if (P.has_chirp) dechirp_stage<T>(P, lds);
+  }
+  store_stage<T, ST>(P, dst, lds, g0, nt);
+}
+
 // FP64 workgroups are capped at 512 threads so that a thread may use up to 256
 // VGPRs (16 staged double-complex values without spilling).
 template <typename T, int LD, int ST>
@@ -582,14 +600,46 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) fused_fft_kernel(
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
   const int g0 = blockIdx.x * P.tpb;
   const int nt = min(P.tpb, P.G - g0);
-  load_stage<T, LD>(P, lds, g0, nt);
-  run_passes<T>(lds, P.seq, P.Lp, P.tpb);
-  if (P.has_mid) {
-    mid_stage<T>(P, lds, g0);
-    run_passes<T>(lds, P.seq, P.Lp, P.tpb);
-    if (P.has_chirp) dechirp_stage<T>(P, lds);
+  block_transform<T, LD, ST>(P, P.src, P.dst, lds, g0, nt);
+}
+
+// Four-step transform of a whole length, one transform per workgroup
+// iteration: the phases (columns + twiddle, rows [+ spectrum product +
+// inverse rows + twiddle, inverse columns + de-chirp]) run back to back in the
+// same workgroup through its own scratch slot, so the intermediate is written
+// and re-read within a few microseconds (L2/MALL resident) instead of
+// round-tripping through HBM between two kernel launches. Phase KParams
+// describe a single transform; src/dst advance by src_dist/dst_dist elements
+// per transform.
+template <typename T, int LD, int ST>
+__device__ void run_phase(const KParams& P, const void* src, void* dst, cx<T>* lds) {
+  for (int g0 = 0; g0 < P.G; g0 += P.tpb) {
+    const int nt = min(P.tpb, P.G - g0);
+    block_transform<T, LD, ST>(P, src, dst, lds, g0, nt);
+    __syncthreads();  // LDS reuse by the next tile; scratch visibility for the next phase
   }
-  store_stage<T, ST>(P, lds, g0, nt);
+}
+
+template <typename T, int NPH>
+__global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024)
+    fourstep_kernel(KParams A, KParams B, KParams C, int batch, long long src_dist, long long dst_dist,
+                    void* scratch, long long slot_elems) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* slot = reinterpret_cast<cx<T>*>(scratch) + (long long)blockIdx.x * slot_elems;
+  for (int tr = blockIdx.x; tr < batch; tr += gridDim.x) {
+    const cx<T>* src = reinterpret_cast<const cx<T>*>(A.src) + (long long)tr * src_dist;
+    if constexpr (NPH == 2) {
+      cx<T>* dst = reinterpret_cast<cx<T>*>(B.dst) + (long long)tr * dst_dist;
+      run_phase<T, LD_PLAIN, ST_TW>(A, src, slot, lds);
+      run_phase<T, LD_PLAIN, ST_PLAIN>(B, slot, dst, lds);
+    } else {
+      cx<T>* dst = reinterpret_cast<cx<T>*>(C.dst) + (long long)tr * dst_dist;
+      run_phase<T, LD_FS_CHIRP, ST_TW>(A, src, slot, lds);
+      run_phase<T, LD_PLAIN, ST_TW>(B, slot, slot, lds);
+      run_phase<T, LD_PLAIN, ST_FS_DECHIRP>(C, slot, dst, lds);
+    }
+  }
 }
 
 // Plain copy used for length-1 transforms (cfft1d.rs:213-214 returns the input
@@ -651,7 +701,49 @@ hipError_t set_fused_lds_limits() {
   }
   CHFFT_COMBOS(CHFFT_ATTR)
 #undef CHFFT_ATTR
+  const void* fs[4] = {reinterpret_cast<const void*>(&fourstep_kernel<float, 2>),
+                       reinterpret_cast<const void*>(&fourstep_kernel<float, 3>),
+                       reinterpret_cast<const void*>(&fourstep_kernel<double, 2>),
+                       reinterpret_cast<const void*>(&fourstep_kernel<double, 3>)};
+  for (const void* f : fs) {
+    hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes);
+    if (r != hipSuccess) e = r;
+  }
   return e;
+}
+
+hipError_t launch_fourstep(bool f64, int nph, const KParams& A, const KParams& B, const KParams& C, int batch,
+                           long long src_dist, long long dst_dist, void* scratch, long long slot_elems, dim3 grid,
+                           dim3 block, size_t lds, hipStream_t st) {
+  if (f64) {
+    if (nph == 2)
+      hipLaunchKernelGGL((fourstep_kernel<double, 2>), grid, block, lds, st, A, B, C, batch, src_dist, dst_dist,
+                         scratch, slot_elems);
+    else
+      hipLaunchKernelGGL((fourstep_kernel<double, 3>), grid, block, lds, st, A, B, C, batch, src_dist, dst_dist,
+                         scratch, slot_elems);
+  } else {
+    if (nph == 2)
+      hipLaunchKernelGGL((fourstep_kernel<float, 2>), grid, block, lds, st, A, B, C, batch, src_dist, dst_dist,
+                         scratch, slot_elems);
+    else
+      hipLaunchKernelGGL((fourstep_kernel<float, 3>), grid, block, lds, st, A, B, C, batch, src_dist, dst_dist,
+                         scratch, slot_elems);
+  }
+  return hipGetLastError();
+}
+
+// Workgroups of the four-step kernel that fit on one CU at once.
+int fourstep_occupancy(bool f64, int nph, int block, size_t lds) {
+  int n = 0;
+  hipError_t e;
+  if (f64)
+    e = nph == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fourstep_kernel<double, 2>, block, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fourstep_kernel<double, 3>, block, lds);
+  else
+    e = nph == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fourstep_kernel<float, 2>, block, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fourstep_kernel<float, 3>, block, lds);
+  return e == hipSuccess ? n : 0;
 }
 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st) {

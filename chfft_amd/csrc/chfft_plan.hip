@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -33,6 +34,10 @@ hipError_t launch_fused(bool f64, int ld, int st, const KParams& P, dim3 grid, d
                         hipStream_t stream);
 hipError_t set_fused_lds_limits();
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
+hipError_t launch_fourstep(bool f64, int nph, const KParams& A, const KParams& B, const KParams& C, int batch,
+                           long long src_dist, long long dst_dist, void* scratch, long long slot_elems, dim3 grid,
+                           dim3 block, size_t lds, hipStream_t stream);
+int fourstep_occupancy(bool f64, int nph, int block, size_t lds);
 
 namespace {
 
@@ -272,8 +277,14 @@ enum Kind { K_C2C = 0, K_R2C = 1, K_C2C2D = 2, K_DCT = 3, K_MDCT = 4 };
 enum Slot { S_IN = 0, S_OUT = 1, S_SCRATCH = 2 };
 
 struct Launch {
+  int kind = 0;  // 0 fused kernel, 1 copy, 2 persistent four-step kernel
   int ld, st;
   KParams P;
+  // kind 2: per-phase descriptors of one transform
+  KParams Q[3];
+  int nph;
+  long long src_dist, dst_dist, slot_elems;
+  int batch;
   dim3 grid, block;
   size_t lds;
   int src, dst;      // Slot
@@ -374,6 +385,12 @@ struct C2CEngine {
   int init(size_t len, bool use_f64) {
     n = len;
     f64 = use_f64;
+    // CHFFT_FOURSTEP=split|persistent overrides the batch-size rule below
+    // (used by the tests to cover both schedules)
+    if (const char* e = getenv("CHFFT_FOURSTEP")) {
+      force_split = strcmp(e, "split") == 0;
+      force_persistent = strcmp(e, "persistent") == 0;
+    }
     if (len == 1) {
       mode = 0;
       return CHFFT_OK;
@@ -405,7 +422,11 @@ struct C2CEngine {
     return CHFFT_OK;
   }
 
-  size_t scratch_elems(size_t batch) const { return (mode == 2 ? n : mode == 3 ? M : 0) * batch; }
+  // scratch of the split (multi-launch) four-step
+  size_t scratch_elems(size_t batch) const {
+    if (persistent_ok(batch)) return 0;
+    return (mode == 2 ? n : mode == 3 ? M : 0) * batch;
+  }
 
   // Appends the launches for `batch` transforms; `inverse` selects the
   // conjugation pair; src/dst slots as given.
@@ -414,7 +435,7 @@ struct C2CEngine {
     const int ci = inverse ? 1 : 0;
     if (mode == 0) {
       Launch l{};
-      l.ld = -1;  // copy
+      l.kind = 1;  // copy
       l.src = src;
       l.dst = dst;
       l.P.G = (int)batch;
@@ -437,6 +458,10 @@ struct C2CEngine {
       l.what = std::string(inner.bluestein ? "fused bluestein N=" + std::to_string(n) + " M " : "fused ") +
                seq_str(inner.seq);
       out.push_back(l);
+      return;
+    }
+    if (persistent_ok(batch)) {
+      emit_persistent(out, batch, inverse, src, dst, dist_in, dist_out);
       return;
     }
     if (mode == 2) {
@@ -519,6 +544,111 @@ struct C2CEngine {
     out.push_back(c);
   }
 
+  // One persistent launch pays off once the batch keeps every CU busy with
+  // whole transforms; smaller batches split the phases over launches so that
+  // all CUs share each transform.
+  bool persistent_ok(size_t batch) const {
+    if (force_split) return false;
+    if (force_persistent) return true;
+    return batch >= (size_t)num_cus();
+  }
+
+  static int num_cus() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) return 256;
+    return n;
+  }
+
+  int fourstep_grid(size_t batch) const {
+    // phases share one workgroup size and LDS size (set in emit_persistent)
+    int occ = fourstep_occupancy(f64, mode == 2 ? 2 : 3, p_block, p_lds);
+    if (occ < 1) occ = 1;
+    long long g = (long long)occ * num_cus();
+    if (g > (long long)batch) g = (long long)batch;
+    return (int)std::max(1LL, g);
+  }
+
+  mutable int p_block = 0;
+  mutable size_t p_lds = 0;
+  bool force_split = false, force_persistent = false;
+
+  void emit_persistent(std::vector<Launch>& out, size_t batch, bool inverse, int src, int dst, long long dist_in,
+                       long long dist_out) const {
+    const int ci = inverse ? 1 : 0;
+    Launch l{};
+    l.kind = 2;
+    l.nph = mode == 2 ? 2 : 3;
+    KParams& A = l.Q[0];
+    KParams& B = l.Q[1];
+    KParams& C = l.Q[2];
+    plain_seq(A, s1, n1);
+    A.G = (int)n2;
+    A.in = io(0, 1, (long long)n2, (int)n2);
+    A.out = A.in;
+    A.colmajor_ld = A.colmajor_st = 1;
+    A.conj_in = ci;
+    A.twN = (long long)(mode == 2 ? n : M);
+    A.tab_a = twN.p;
+    plain_seq(B, s2, n2);
+    B.G = (int)n1;
+    B.in = io(0, (long long)n2, 1, (int)n1);
+    if (mode == 2) {
+      B.out = io(0, 1, (long long)n1, (int)n1);
+      B.colmajor_st = 1;
+      B.conj_out = ci;
+    } else {
+      A.N = (long long)n;
+      A.chirp = chirp.p;
+      B.out = B.in;
+      B.has_mid = 1;
+      B.kspec = kspec.p;
+      B.kc = 1;
+      B.ke = (long long)n1;
+      B.twN = (long long)M;
+      B.tab_a = twN.p;
+      plain_seq(C, s1, n1);
+      C.G = (int)n2;
+      C.in = io(0, 1, (long long)n2, (int)n2);
+      C.out = C.in;
+      C.colmajor_ld = C.colmajor_st = 1;
+      C.N = (long long)n;
+      C.chirp = chirp.p;
+      C.conj_out = ci;
+    }
+    Geometry ga = choose_geometry((int)n1, 1 << 20, true, f64);
+    Geometry gb = choose_geometry((int)n2, 1 << 20, true, f64);
+    A.tpb = ga.tpb;
+    A.Lp = ga.Lp;
+    B.tpb = gb.tpb;
+    B.Lp = gb.Lp;
+    C.tpb = ga.tpb;
+    C.Lp = ga.Lp;
+    p_block = std::max(ga.block, gb.block);
+    p_lds = std::max(ga.lds, gb.lds);
+    l.block = dim3((unsigned)p_block);
+    l.lds = p_lds;
+    l.batch = (int)batch;
+    l.src_dist = dist_in;
+    l.dst_dist = dist_out;
+    l.slot_elems = (long long)(mode == 2 ? n : M);
+    l.grid = dim3((unsigned)fourstep_grid(batch));
+    l.src = src;
+    l.dst = dst;
+    l.scaled = 1;
+    l.scale_mul = 1.0;
+    l.what = std::string(mode == 2 ? "persistent four-step " : "persistent bluestein N=" + std::to_string(n) + " ") +
+             "columns " + seq_str(s1) + " x rows " + seq_str(s2) + " (one transform per workgroup iteration)";
+    out.push_back(l);
+  }
+
+  size_t persistent_scratch_elems(const std::vector<Launch>& L) const {
+    size_t need = 0;
+    for (const Launch& l : L)
+      if (l.kind == 2) need = std::max(need, (size_t)l.grid.x * (size_t)l.slot_elems);
+    return need;
+  }
+
   static void plain_seq(KParams& P, const SeqDesc& s, size_t L) {
     P.seq = s;
     P.Lfft = (int)L;
@@ -592,6 +722,8 @@ int finish_plan(chfft_plan_s* p, chfft_plan* out) {
   size_t need = 0;
   if (p->kind == K_C2C) need = p->rows.scratch_elems(p->batch);
   if (p->kind == K_C2C2D) need = p->rows.scratch_elems(p->batch * p->m);
+  need = std::max(need, p->rows.persistent_scratch_elems(p->fwd));
+  need = std::max(need, p->rows.persistent_scratch_elems(p->bwd));
   int rc = dev_alloc(p->scratch, need * esz_of(p->f64));
   if (rc) {
     delete p;
@@ -658,6 +790,8 @@ int chfft_plan_c2c_2d(chfft_plan* plan, size_t len_m, size_t len_n, size_t batch
     delete p;
     return rc;
   }
+  p->rows.force_split = true;  // the column pass below continues from split launches
+  p->rows.force_persistent = false;
   if (len_m > 1 && !fusable(len_m, p->f64)) {
     delete p;
     return CHFFT_ERR_UNSUPPORTED;
@@ -956,7 +1090,18 @@ int chfft_exec_range(chfft_plan plan, const void* in, void* out, int method, int
     const Launch& l0 = L[li];
     const void* slots_src[3] = {in, out, plan->scratch.p};
     void* slots_dst[3] = {const_cast<void*>(in), out, plan->scratch.p};
-    if (l0.ld < 0) {  // length-1 copy
+    if (l0.kind == 2) {
+      KParams A = l0.Q[0], B = l0.Q[1], C = l0.Q[2];
+      A.src = slots_src[l0.src];
+      (l0.nph == 2 ? B : C).dst = slots_dst[l0.dst];
+      A.scale = B.scale = C.scale = 1.0;
+      (l0.nph == 2 ? B : C).scale = scale * l0.scale_mul;
+      if (launch_fourstep(plan->f64, l0.nph, A, B, C, l0.batch, l0.src_dist, l0.dst_dist, plan->scratch.p,
+                          l0.slot_elems, l0.grid, l0.block, l0.lds, st) != hipSuccess)
+        return CHFFT_ERR_HIP;
+      continue;
+    }
+    if (l0.kind == 1) {  // length-1 copy
       size_t ib, ob;
       io_bytes(plan, inverse, &ib, &ob);
       if (slots_src[l0.src] != slots_dst[l0.dst] &&
