@@ -14,6 +14,9 @@ constexpr int kEmax = 16;
 // Workgroup size limits of the fused kernel (FP32 / FP64).
 constexpr int kMaxThreadsF32 = 1024;
 constexpr int kMaxThreadsF64 = 512;
+// ... and of the persistent four-step kernel (more live state per thread)
+constexpr int kMaxThreadsFsF32 = 512;
+constexpr int kMaxThreadsFsF64 = 256;
 // LDS per workgroup on gfx950.
 constexpr size_t kLdsBytes = 160 * 1024;
 
@@ -24,6 +27,7 @@ struct PassDesc {
   int radix;
   int p;
   int tws;
+  unsigned pmagic;  // ceil(2^32 / p): i / p == umulhi(i, pmagic) for i < 2^16
 };
 
 struct SeqDesc {

@@ -208,35 +208,60 @@ __device__ __forceinline__ void dft(cx<T>* x) {
   else if constexpr (R == 16) dft16(x);
 }
 
+// ------------------------------------------------------------------ thread map
+// A workgroup of tpb * tpt threads owns tpb transforms. For LDS passes and
+// row-major global access, thread (t, u) = (tid / tpt, tid % tpt) works on
+// transform t; for column-major global access (consecutive threads on
+// consecutive transforms, so that strided columns are read in full cache
+// lines) it is (tid % tpb, tid / tpb). Both are computed once per thread.
+struct TMap {
+  int tpt;
+  int tr, ur;  // row-major
+  int tc, uc;  // column-major
+};
+
+__device__ __forceinline__ TMap make_map(int tpb) {
+  TMap m;
+  m.tpt = blockDim.x / tpb;
+  m.tr = threadIdx.x / m.tpt;
+  m.ur = threadIdx.x - m.tr * m.tpt;
+  m.tc = threadIdx.x % tpb;
+  m.uc = threadIdx.x / tpb;
+  return m;
+}
+
+// floor(i / p) for i < 2^16, p <= 2^14, with pm = ceil(2^32 / p).
+__device__ __forceinline__ int qdiv(int i, unsigned pm) { return (int)__umulhi((unsigned)i, pm); }
+
 // ------------------------------------------------------------------ LDS passes
 // One Stockham pass (span p -> p*R) over all tpb sequences of the workgroup.
 // Butterfly i of a sequence reads x[i + j*L/R], multiplies input j by
 // W_{pR}^{j*k} (k = i mod p), and writes X_j to ((i-k)*R + k + j*p).
+// Inputs of all of a thread's butterflies are staged in registers between the
+// read barrier and the write, so one LDS buffer suffices.
 template <typename T, int R>
-__device__ __forceinline__ void lds_pass(cx<T>* lds, int L, int Lp, int tpb, int p, int tws,
+__device__ __forceinline__ void lds_pass(cx<T>* lds, const TMap& m, int L, int Lp, const PassDesc& pd,
                                          const cx<T>* __restrict__ tw) {
   constexpr int BPT = (kEmax + R - 1) / R;
   const int nbt = L / R;
-  const int nb = tpb * nbt;
+  const int p = pd.p;
   cx<T> x[BPT][R];
   int obase[BPT];
+  cx<T>* base = lds + m.tr * Lp;
 #pragma unroll
   for (int q = 0; q < BPT; q++) {
-    const int b = threadIdx.x + q * blockDim.x;
+    const int i = m.ur + q * m.tpt;
     obase[q] = -1;
-    if (b < nb) {
-      const int t = b / nbt;
-      const int i = b - t * nbt;
-      const int k = i % p;
-      const cx<T>* s = lds + t * Lp + i;
+    if (i < nbt) {
+      const int k = (p == 1) ? 0 : i - qdiv(i, pd.pmagic) * p;
 #pragma unroll
-      for (int j = 0; j < R; j++) x[q][j] = s[j * nbt];
+      for (int j = 0; j < R; j++) x[q][j] = base[i + j * nbt];
       if (p > 1) {
-        const int step = k * tws;
+        const int step = k * pd.tws;
 #pragma unroll
         for (int j = 1; j < R; j++) x[q][j] = x[q][j] * tw[j * step];
       }
-      obase[q] = t * Lp + (i - k) * R + k;
+      obase[q] = (i - k) * R + k;
     }
   }
   __syncthreads();
@@ -244,7 +269,7 @@ __device__ __forceinline__ void lds_pass(cx<T>* lds, int L, int Lp, int tpb, int
   for (int q = 0; q < BPT; q++) {
     if (obase[q] >= 0) {
       dft<T, R>(x[q]);
-      cx<T>* d = lds + obase[q];
+      cx<T>* d = base + obase[q];
 #pragma unroll
       for (int j = 0; j < R; j++) d[j * p] = x[q][j];
     }
@@ -253,47 +278,68 @@ __device__ __forceinline__ void lds_pass(cx<T>* lds, int L, int Lp, int tpb, int
 }
 
 template <typename T>
-__device__ void run_passes(cx<T>* lds, const SeqDesc& s, int Lp, int tpb) {
+__device__ void run_passes(cx<T>* lds, const TMap& m, const SeqDesc& s, int Lp) {
   const cx<T>* tw = reinterpret_cast<const cx<T>*>(s.tw);
   for (int q = 0; q < s.npass; q++) {
-    const PassDesc pd = s.pass[q];
+    const PassDesc& pd = s.pass[q];
     switch (pd.radix) {
-      case 2: lds_pass<T, 2>(lds, s.L, Lp, tpb, pd.p, pd.tws, tw); break;
-      case 3: lds_pass<T, 3>(lds, s.L, Lp, tpb, pd.p, pd.tws, tw); break;
-      case 4: lds_pass<T, 4>(lds, s.L, Lp, tpb, pd.p, pd.tws, tw); break;
-      case 5: lds_pass<T, 5>(lds, s.L, Lp, tpb, pd.p, pd.tws, tw); break;
-      case 7: lds_pass<T, 7>(lds, s.L, Lp, tpb, pd.p, pd.tws, tw); break;
-      case 8: lds_pass<T, 8>(lds, s.L, Lp, tpb, pd.p, pd.tws, tw); break;
-      case 16: lds_pass<T, 16>(lds, s.L, Lp, tpb, pd.p, pd.tws, tw); break;
+      case 2: lds_pass<T, 2>(lds, m, s.L, Lp, pd, tw); break;
+      case 3: lds_pass<T, 3>(lds, m, s.L, Lp, pd, tw); break;
+      case 4: lds_pass<T, 4>(lds, m, s.L, Lp, pd, tw); break;
+      case 5: lds_pass<T, 5>(lds, m, s.L, Lp, pd, tw); break;
+      case 7: lds_pass<T, 7>(lds, m, s.L, Lp, pd, tw); break;
+      case 8: lds_pass<T, 8>(lds, m, s.L, Lp, pd, tw); break;
+      case 16: lds_pass<T, 16>(lds, m, s.L, Lp, pd, tw); break;
       default: break;
     }
   }
 }
 
 // ------------------------------------------------------------------ operators
-__device__ __forceinline__ long long io_addr(const IoDesc& d, int g, long long idx) {
+// Offset of transform g (its element 0) and its column index under IoDesc d.
+__device__ __forceinline__ long long io_base(const IoDesc& d, int g, int* col) {
   const int b = g / d.ncols;
   const int c = g - b * d.ncols;
-  return (long long)b * d.bdist + (long long)c * d.cstride + idx * d.estride;
+  *col = c;
+  return (long long)b * d.bdist + (long long)c * d.cstride;
 }
 
-// Element n (of Lfft) of transform g for the load stage, before conjugation.
+// Per-transform state of the load operator (computed once per thread).
 template <typename T, int LD>
-__device__ __forceinline__ cx<T> load_elem(const KParams& P, const void* src, int g, int n) {
+struct Loader {
+  const cx<T>* cs;  // complex row / column base
+  const T* rs;      // real row base
+  long long seq0;   // FS_CHIRP: sequence index of element 0 (= column * cstride)
+  long long brow;   // FS_CHIRP: batch base
+};
+
+template <typename T, int LD>
+__device__ __forceinline__ Loader<T, LD> make_loader(const KParams& P, const void* src, int g) {
+  Loader<T, LD> L;
+  int c;
+  const long long base = io_base(P.in, g, &c);
+  L.cs = reinterpret_cast<const cx<T>*>(src) + base;
+  L.rs = reinterpret_cast<const T*>(src) + base;
+  L.seq0 = (long long)c * P.in.cstride;
+  L.brow = base - L.seq0;
+  return L;
+}
+
+// Element n of the load image of one transform, before conjugation.
+template <typename T, int LD>
+__device__ __forceinline__ cx<T> load_elem(const KParams& P, const Loader<T, LD>& ld, int n) {
   const cx<T> zero = mk(T(0), T(0));
   if constexpr (LD == LD_PLAIN) {
-    return reinterpret_cast<const cx<T>*>(src)[io_addr(P.in, g, n)];
+    return ld.cs[(long long)n * P.in.estride];
   } else if constexpr (LD == LD_FS_CHIRP) {
-    const int b = g / P.in.ncols;
-    const int c = g - b * P.in.ncols;
-    const long long s = (long long)c * P.in.cstride + (long long)n * P.in.estride;
+    const long long s = ld.seq0 + (long long)n * P.in.estride;
     if (s >= P.N) return zero;
-    cx<T> v = reinterpret_cast<const cx<T>*>(src)[(long long)b * P.in.bdist + s];
+    cx<T> v = reinterpret_cast<const cx<T>*>(ld.cs - ld.seq0)[s];
     if (P.conj_in) v = cj(v);  // inverse: conjugate the input, not the chirp
     return v * reinterpret_cast<const cx<T>*>(P.chirp)[s];
   } else if constexpr (LD == LD_R2C_BACK) {
     // rfft1d.rs:161-177 (complex input, N/2+1 bins per row)
-    const cx<T>* X = reinterpret_cast<const cx<T>*>(src) + (long long)g * P.in.bdist;
+    const cx<T>* X = ld.cs;
     const cx<T>* bc = reinterpret_cast<const cx<T>*>(P.tab_a);
     const int N = (int)P.N, h = N >> 1, q = (N + 3) >> 2;
     if (n == 0) {
@@ -307,7 +353,7 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, const void* src, in
     return (n < q) ? xi - x : xh + cj(x);
   } else if constexpr (LD == LD_DCT2) {
     // dct1d.rs:245-263 (real input)
-    const T* x = reinterpret_cast<const T*>(src) + (long long)g * P.in.bdist;
+    const T* x = ld.rs;
     const int N = (int)P.N, h = N >> 1, q = h >> 1;
     const T o = (T)P.ortho;
     if (q == 0) return mk(x[0] * o, x[1]);
@@ -318,7 +364,7 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, const void* src, in
     return mk(x[4 * i + 3], x[4 * i + 1]);
   } else if constexpr (LD == LD_DCT3) {
     // dct1d.rs:367-391 (real input)
-    const T* x = reinterpret_cast<const T*>(src) + (long long)g * P.in.bdist;
+    const T* x = ld.rs;
     const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
     const cx<T>* om = reinterpret_cast<const cx<T>*>(P.tab_b);
     const int N = (int)P.N, h = N >> 1, q = (N + 3) >> 2;
@@ -335,7 +381,7 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, const void* src, in
     return (n < q) ? j - xx : cj(k) + cj(xx);
   } else if constexpr (LD == LD_MDCT) {
     // mdct1d.rs:206-230 (frames `bdist` samples apart)
-    const T* x = reinterpret_cast<const T*>(src) + (long long)g * P.in.bdist;
+    const T* x = ld.rs;
     const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
     const T* w = reinterpret_cast<const T*>(P.tab_b);
     const int N = (int)P.N, q = N >> 2, h = N >> 1, q3 = h + q, q5 = h + q3, lo = q - (q >> 1);
@@ -347,7 +393,7 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, const void* src, in
       v = mk(x[q3 - 1 - m] * w[q3 - 1 - m] - x[m - q] * w[m - q], x[q + m] * w[q + m] + x[q5 - 1 - m] * w[q5 - 1 - m]);
     return v * tw[n];
   } else {  // LD_IMDCT, mdct1d.rs:261-265
-    const T* c = reinterpret_cast<const T*>(src) + (long long)g * P.in.bdist;
+    const T* c = ld.rs;
     const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
     const int h = (int)(P.N >> 1);
     return mk(c[2 * n], c[h - 1 - 2 * n]) * scl(tw[n], T(-2));
@@ -355,28 +401,26 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, const void* src, in
 }
 
 template <typename T, int LD>
-__device__ void load_stage(const KParams& P, const void* src, cx<T>* lds, int g0, int nt) {
-  const int Lf = P.Lfft;
-  const int total = P.tpb * Lf;
+__device__ void load_stage(const KParams& P, const TMap& m, const void* src, cx<T>* lds, int g0, int nt) {
+  const int t = P.colmajor_ld ? m.tc : m.tr;
+  const int u = P.colmajor_ld ? m.uc : m.ur;
   const cx<T>* chirp = reinterpret_cast<const cx<T>*>(P.chirp);
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    int t, n;
-    if (P.colmajor_ld) {
-      t = e % P.tpb;
-      n = e / P.tpb;
-    } else {
-      t = e / Lf;
-      n = e - t * Lf;
-    }
-    cx<T> v = mk(T(0), T(0));
-    if (t < nt && n < P.L) {  // n >= L only under a fused Bluestein: zero padding
-      v = load_elem<T, LD>(P, src, g0 + t, n);
-      if constexpr (LD != LD_FS_CHIRP) {
-        if (P.conj_in) v = cj(v);
+  cx<T>* row = lds + t * P.Lp;
+  if (t < nt) {
+    const Loader<T, LD> ld = make_loader<T, LD>(P, src, g0 + t);
+    for (int n = u; n < P.Lfft; n += m.tpt) {
+      cx<T> v = mk(T(0), T(0));
+      if (n < P.L) {  // n >= L only under a fused Bluestein: zero padding
+        v = load_elem<T, LD>(P, ld, n);
+        if constexpr (LD != LD_FS_CHIRP) {
+          if (P.conj_in) v = cj(v);
+        }
+        if (P.has_chirp) v = v * chirp[n];
       }
-      if (P.has_chirp && n < P.L) v = v * chirp[n];
+      row[n] = v;
     }
-    lds[t * P.Lp + n] = v;
+  } else {
+    for (int n = u; n < P.Lfft; n += m.tpt) row[n] = mk(T(0), T(0));
   }
   __syncthreads();
 }
@@ -384,101 +428,81 @@ __device__ void load_stage(const KParams& P, const void* src, cx<T>* lds, int g0
 // Bluestein spectrum product (chirpz.rs:115-117) followed by a conjugation so
 // that the inverse transform reuses the forward passes.
 template <typename T>
-__device__ void mid_stage(const KParams& P, cx<T>* lds, int g0) {
-  const int Lf = P.Lfft;
-  const int total = P.tpb * Lf;
+__device__ void mid_stage(const KParams& P, const TMap& m, cx<T>* lds, int g0) {
   const cx<T>* K = reinterpret_cast<const cx<T>*>(P.kspec);
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int t = e / Lf;
-    const int n = e - t * Lf;
-    const int g = g0 + t;
-    const int c = g % P.in.ncols;
-    const long long ki = (long long)c * P.kc + (long long)n * P.ke;
-    cx<T>& a = lds[t * P.Lp + n];
-    a = cj(a * K[ki]);
-  }
+  const int c = (g0 + m.tr) % P.in.ncols;
+  const long long kb = (long long)c * P.kc;
+  cx<T>* row = lds + m.tr * P.Lp;
+  for (int n = m.ur; n < P.Lfft; n += m.tpt) row[n] = cj(row[n] * K[kb + (long long)n * P.ke]);
   __syncthreads();
 }
 
 // Fused Bluestein de-chirp (chirpz.rs:127-145 with the conjugation undone).
 template <typename T>
-__device__ void dechirp_stage(const KParams& P, cx<T>* lds) {
+__device__ void dechirp_stage(const KParams& P, const TMap& m, cx<T>* lds) {
   const cx<T>* chirp = reinterpret_cast<const cx<T>*>(P.chirp);
-  const int total = P.tpb * P.L;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int t = e / P.L;
-    const int n = e - t * P.L;
-    cx<T>& a = lds[t * P.Lp + n];
-    a = cj(a) * chirp[n];
-  }
+  cx<T>* row = lds + m.tr * P.Lp;
+  for (int n = m.ur; n < P.L; n += m.tpt) row[n] = cj(row[n]) * chirp[n];
   __syncthreads();
 }
 
-template <typename T>
-__device__ __forceinline__ cx<T> getz(const KParams& P, const cx<T>* lds, int t, int k) {
-  cx<T> z = lds[t * P.Lp + k];
-  return P.conj_out ? cj(z) : z;
-}
-
 template <typename T, int ST>
-__device__ void store_stage(const KParams& P, void* dst, const cx<T>* lds, int g0, int nt) {
-  const T s = (T)P.scale;
+__device__ void store_stage(const KParams& P, const TMap& m, void* dst, const cx<T>* lds, int g0, int nt,
+                            double scale) {
+  const T s = (T)scale;
   if constexpr (ST == ST_PLAIN || ST == ST_TW || ST == ST_FS_DECHIRP) {
-    const int L = P.L;
-    const int total = P.tpb * L;
-    cx<T>* out = reinterpret_cast<cx<T>*>(dst);
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-      int t, k;
-      if (P.colmajor_st) {
-        t = e % P.tpb;
-        k = e / P.tpb;
-      } else {
-        t = e / L;
-        k = e - t * L;
-      }
-      if (t >= nt) continue;
-      const int g = g0 + t;
-      cx<T> v = lds[t * P.Lp + k];
+    const int t = P.colmajor_st ? m.tc : m.tr;
+    const int u = P.colmajor_st ? m.uc : m.ur;
+    if (t >= nt) return;
+    const cx<T>* row = lds + t * P.Lp;
+    int c;
+    const long long base = io_base(P.out, g0 + t, &c);
+    cx<T>* out = reinterpret_cast<cx<T>*>(dst) + base;
+    const long long es = P.out.estride;
+    for (int k = u; k < P.L; k += m.tpt) {
+      cx<T> v = row[k];
       if constexpr (ST == ST_PLAIN) {
         if (P.conj_out) v = cj(v);
-        out[io_addr(P.out, g, k)] = scl(v, s);
+        out[k * es] = scl(v, s);
       } else if constexpr (ST == ST_TW) {
-        const int c = g % P.out.ncols;
-        const long long m = ((long long)c * k) % P.twN;
-        v = v * reinterpret_cast<const cx<T>*>(P.tab_a)[m];
+        // c < columns and k < rows, so c * k < twN: no reduction needed
+        v = v * reinterpret_cast<const cx<T>*>(P.tab_a)[c * k];
         if (P.conj_out) v = cj(v);
-        out[io_addr(P.out, g, k)] = scl(v, s);
+        out[k * es] = scl(v, s);
       } else {
-        const int b = g / P.out.ncols;
-        const int c = g - b * P.out.ncols;
-        const long long j = (long long)c * P.out.cstride + (long long)k * P.out.estride;
+        const long long j = (long long)c * P.out.cstride + (long long)k * es;
         if (j < P.N) {
           v = cj(v) * reinterpret_cast<const cx<T>*>(P.chirp)[j];
           if (P.conj_out) v = cj(v);
-          out[(long long)b * P.out.bdist + j] = scl(v, s);
+          reinterpret_cast<cx<T>*>(dst)[base - (long long)c * P.out.cstride + j] = scl(v, s);
         }
       }
     }
-  } else if constexpr (ST == ST_R2C) {
+    return;
+  }
+  const int t = m.tr;
+  if (t >= nt) return;
+  const cx<T>* row = lds + t * P.Lp;
+  auto Z = [&](int k) {
+    cx<T> z = row[k];
+    return P.conj_out ? cj(z) : z;
+  };
+  const long long obase = (long long)(g0 + t) * P.out.bdist;
+  if constexpr (ST == ST_R2C) {
     // rfft1d.rs:134-148: N/2+1 bins per row
     const int N = (int)P.N, h = N >> 1, q = (N + 3) >> 2;
     const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
-    const int per = h + 1;
-    const int total = P.tpb * per;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-      const int t = e / per;
-      const int r = e - t * per;
-      if (t >= nt) continue;
-      cx<T>* out = reinterpret_cast<cx<T>*>(dst) + (long long)(g0 + t) * P.out.bdist;
+    cx<T>* out = reinterpret_cast<cx<T>*>(dst) + obase;
+    for (int r = m.ur; r <= h; r += m.tpt) {
       cx<T> v;
       if (r == 0 || r == h) {
-        const cx<T> z = getz(P, lds, t, 0);
+        const cx<T> z = Z(0);
         v = mk(r == 0 ? z.re + z.im : z.re - z.im, T(0));
       } else if ((N & 3) == 0 && r == q) {
-        v = cj(getz(P, lds, t, q));
+        v = cj(Z(q));
       } else {
         const int i = (r < q) ? r : h - r;
-        const cx<T> zi = getz(P, lds, t, i), zh = getz(P, lds, t, h - i);
+        const cx<T> zi = Z(i), zh = Z(h - i);
         const cx<T> x = co[i] * (zi - cj(zh));
         v = (r < q) ? zi - x : zh + cj(x);
       }
@@ -489,20 +513,15 @@ __device__ void store_stage(const KParams& P, void* dst, const cx<T>* lds, int g
     const int N = (int)P.N, h = N >> 1, q2 = (N + 3) >> 2;
     const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
     const cx<T>* om = reinterpret_cast<const cx<T>*>(P.tab_b);
-    const int per = q2 + 1;
-    const int total = P.tpb * per;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-      const int t = e / per;
-      const int i = e - t * per;
-      if (t >= nt) continue;
-      T* ret = reinterpret_cast<T*>(dst) + (long long)(g0 + t) * P.out.bdist;
+    T* ret = reinterpret_cast<T*>(dst) + obase;
+    for (int i = m.ur; i <= q2; i += m.tpt) {
       if (i == 0) {
-        const cx<T> z = getz(P, lds, t, 0);
+        const cx<T> z = Z(0);
         ret[0] = (z.re + z.im) * s;
         ret[h] = (z.re - z.im) * om[h].re * s;
       } else if (i < q2) {
-        const cx<T> zi = getz(P, lds, t, i);
-        const cx<T> wc = cj(getz(P, lds, t, h - i));
+        const cx<T> zi = Z(i);
+        const cx<T> wc = cj(Z(h - i));
         const cx<T> x = co[i] * (zi - wc);
         const cx<T> j = cj(zi - x) * om[i];
         ret[i] = j.re * s;
@@ -511,7 +530,7 @@ __device__ void store_stage(const KParams& P, void* dst, const cx<T>* lds, int g
         ret[h - i] = k.re * s;
         ret[h + i] = k.im * s;
       } else if ((N & 3) == 0) {
-        const cx<T> x = getz(P, lds, t, q2) * om[q2];
+        const cx<T> x = Z(q2) * om[q2];
         ret[q2] = x.re * s;
         ret[N - q2] = x.im * s;
       }
@@ -520,15 +539,11 @@ __device__ void store_stage(const KParams& P, void* dst, const cx<T>* lds, int g
     // dct1d.rs:395-407
     const int N = (int)P.N, h = N >> 1, q2 = h >> 1;
     const int per = q2 + (h & 1);
-    const int total = P.tpb * per;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-      const int t = e / per;
-      const int i = e - t * per;
-      if (t >= nt) continue;
-      T* ret = reinterpret_cast<T*>(dst) + (long long)(g0 + t) * P.out.bdist;
-      const cx<T> a = getz(P, lds, t, i);
+    T* ret = reinterpret_cast<T*>(dst) + obase;
+    for (int i = m.ur; i < per; i += m.tpt) {
+      const cx<T> a = Z(i);
       if (i < q2) {
-        const cx<T> b = getz(P, lds, t, h - 1 - i);
+        const cx<T> b = Z(h - 1 - i);
         ret[4 * i] = a.re * s;
         ret[4 * i + 1] = b.im * s;
         ret[4 * i + 2] = a.im * s;
@@ -538,17 +553,14 @@ __device__ void store_stage(const KParams& P, void* dst, const cx<T>* lds, int g
         ret[4 * i + 1] = a.im * s;
       }
     }
+    (void)N;
   } else if constexpr (ST == ST_MDCT) {
     // mdct1d.rs:234-240
-    const int N = (int)P.N, q = N >>2, h = N >> 1;
+    const int N = (int)P.N, q = N >> 2, h = N >> 1;
     const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
-    const int total = P.tpb * q;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-      const int t = e / q;
-      const int i = e - t * q;
-      if (t >= nt) continue;
-      T* ret = reinterpret_cast<T*>(dst) + (long long)(g0 + t) * P.out.bdist;
-      const cx<T> v = getz(P, lds, t, i) * scl(tw[i], s);
+    T* ret = reinterpret_cast<T*>(dst) + obase;
+    for (int i = m.ur; i < q; i += m.tpt) {
+      const cx<T> v = Z(i) * scl(tw[i], s);
       ret[2 * i] = -v.re;
       ret[h - 1 - 2 * i] = v.im;
     }
@@ -556,22 +568,18 @@ __device__ void store_stage(const KParams& P, void* dst, const cx<T>* lds, int g
     const int N = (int)P.N, q = N >> 2, h = N >> 1, q3 = h + q, q5 = h + q3, lo = q - (q >> 1);
     const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
     const T* w = reinterpret_cast<const T*>(P.tab_b);
-    const int total = P.tpb * q;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-      const int t = e / q;
-      const int i = e - t * q;
-      if (t >= nt) continue;
-      T* ret = reinterpret_cast<T*>(dst) + (long long)(g0 + t) * P.out.bdist;
-      const cx<T> v = getz(P, lds, t, i) * scl(tw[i], s);
-      const int m = 2 * i;
-      ret[q3 - 1 - m] = v.re * w[q3 - 1 - m];
-      ret[q + m] = -v.im * w[q + m];
+    T* ret = reinterpret_cast<T*>(dst) + obase;
+    for (int i = m.ur; i < q; i += m.tpt) {
+      const cx<T> v = Z(i) * scl(tw[i], s);
+      const int mm = 2 * i;
+      ret[q3 - 1 - mm] = v.re * w[q3 - 1 - mm];
+      ret[q + mm] = -v.im * w[q + mm];
       if (i < lo) {
-        ret[q3 + m] = v.re * w[q3 + m];
-        ret[q - 1 - m] = v.im * w[q - 1 - m];
+        ret[q3 + mm] = v.re * w[q3 + mm];
+        ret[q - 1 - mm] = v.im * w[q - 1 - mm];
       } else {
-        ret[m - q] = -v.re * w[m - q];
-        ret[q5 - 1 - m] = -v.im * w[q5 - 1 - m];
+        ret[mm - q] = -v.re * w[mm - q];
+        ret[q5 - 1 - mm] = -v.im * w[q5 - 1 - mm];
       }
     }
   }
@@ -581,15 +589,16 @@ __device__ void store_stage(const KParams& P, void* dst, const cx<T>* lds, int g
 // store. The store stage is not followed by a barrier.
 template <typename T, int LD, int ST>
 __device__ __forceinline__ void block_transform(const KParams& P, const void* src, void* dst, cx<T>* lds, int g0,
-                                                int nt) {
-  load_stage<T, LD>(P, src, lds, g0, nt);
-  run_passes<T>(lds, P.seq, P.Lp, P.tpb);
+                                                int nt, double scale) {
+  const TMap m = make_map(P.tpb);
+  load_stage<T, LD>(P, m, src, lds, g0, nt);
+  run_passes<T>(lds, m, P.seq, P.Lp);
   if (P.has_mid) {
-    mid_stage<T>(P, lds, g0);
-    run_passes<T>(lds, P.seq, P.Lp, P.tpb);
-    if (P.has_chirp) dechirp_stage<T>(P, lds);
+    mid_stage<T>(P, m, lds, g0);
+    run_passes<T>(lds, m, P.seq, P.Lp);
+    if (P.has_chirp) dechirp_stage<T>(P, m, lds);
   }
-  store_stage<T, ST>(P, dst, lds, g0, nt);
+  store_stage<T, ST>(P, m, dst, lds, g0, nt, scale);
 }
 
 // FP64 workgroups are capped at 512 threads so that a thread may use up to 256
@@ -600,7 +609,7 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) fused_fft_kernel(
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
   const int g0 = blockIdx.x * P.tpb;
   const int nt = min(P.tpb, P.G - g0);
-  block_transform<T, LD, ST>(P, P.src, P.dst, lds, g0, nt);
+  block_transform<T, LD, ST>(P, P.src, P.dst, lds, g0, nt, P.scale);
 }
 
 // Four-step transform of a whole length, one transform per workgroup
@@ -612,32 +621,33 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024) fused_fft_kernel(
 // describe a single transform; src/dst advance by src_dist/dst_dist elements
 // per transform.
 template <typename T, int LD, int ST>
-__device__ void run_phase(const KParams& P, const void* src, void* dst, cx<T>* lds) {
+__device__ void run_phase(const KParams& P, const void* src, void* dst, cx<T>* lds, double scale) {
   for (int g0 = 0; g0 < P.G; g0 += P.tpb) {
     const int nt = min(P.tpb, P.G - g0);
-    block_transform<T, LD, ST>(P, src, dst, lds, g0, nt);
+    block_transform<T, LD, ST>(P, src, dst, lds, g0, nt, scale);
     __syncthreads();  // LDS reuse by the next tile; scratch visibility for the next phase
   }
 }
 
+// Q points to the phase descriptors in device memory (uploaded once per plan);
+// scale applies to the last phase.
 template <typename T, int NPH>
-__global__ void __launch_bounds__(sizeof(T) == 8 ? 512 : 1024)
-    fourstep_kernel(KParams A, KParams B, KParams C, int batch, long long src_dist, long long dst_dist,
-                    void* scratch, long long slot_elems) {
+__global__ void __launch_bounds__(sizeof(T) == 8 ? kMaxThreadsFsF64 : kMaxThreadsFsF32)
+    fourstep_kernel(const KParams* __restrict__ Q, const void* in, void* out, double scale, int batch,
+                    long long src_dist, long long dst_dist, void* scratch, long long slot_elems) {
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
   cx<T>* slot = reinterpret_cast<cx<T>*>(scratch) + (long long)blockIdx.x * slot_elems;
   for (int tr = blockIdx.x; tr < batch; tr += gridDim.x) {
-    const cx<T>* src = reinterpret_cast<const cx<T>*>(A.src) + (long long)tr * src_dist;
+    const cx<T>* src = reinterpret_cast<const cx<T>*>(in) + (long long)tr * src_dist;
+    cx<T>* dst = reinterpret_cast<cx<T>*>(out) + (long long)tr * dst_dist;
     if constexpr (NPH == 2) {
-      cx<T>* dst = reinterpret_cast<cx<T>*>(B.dst) + (long long)tr * dst_dist;
-      run_phase<T, LD_PLAIN, ST_TW>(A, src, slot, lds);
-      run_phase<T, LD_PLAIN, ST_PLAIN>(B, slot, dst, lds);
+      run_phase<T, LD_PLAIN, ST_TW>(Q[0], src, slot, lds, 1.0);
+      run_phase<T, LD_PLAIN, ST_PLAIN>(Q[1], slot, dst, lds, scale);
     } else {
-      cx<T>* dst = reinterpret_cast<cx<T>*>(C.dst) + (long long)tr * dst_dist;
-      run_phase<T, LD_FS_CHIRP, ST_TW>(A, src, slot, lds);
-      run_phase<T, LD_PLAIN, ST_TW>(B, slot, slot, lds);
-      run_phase<T, LD_PLAIN, ST_FS_DECHIRP>(C, slot, dst, lds);
+      run_phase<T, LD_FS_CHIRP, ST_TW>(Q[0], src, slot, lds, 1.0);
+      run_phase<T, LD_PLAIN, ST_TW>(Q[1], slot, slot, lds, 1.0);
+      run_phase<T, LD_PLAIN, ST_FS_DECHIRP>(Q[2], slot, dst, lds, scale);
     }
   }
 }
@@ -712,23 +722,23 @@ hipError_t set_fused_lds_limits() {
   return e;
 }
 
-hipError_t launch_fourstep(bool f64, int nph, const KParams& A, const KParams& B, const KParams& C, int batch,
+hipError_t launch_fourstep(bool f64, int nph, const KParams* Q, const void* in, void* out, double scale, int batch,
                            long long src_dist, long long dst_dist, void* scratch, long long slot_elems, dim3 grid,
                            dim3 block, size_t lds, hipStream_t st) {
   if (f64) {
     if (nph == 2)
-      hipLaunchKernelGGL((fourstep_kernel<double, 2>), grid, block, lds, st, A, B, C, batch, src_dist, dst_dist,
-                         scratch, slot_elems);
+      hipLaunchKernelGGL((fourstep_kernel<double, 2>), grid, block, lds, st, Q, in, out, scale, batch, src_dist,
+                         dst_dist, scratch, slot_elems);
     else
-      hipLaunchKernelGGL((fourstep_kernel<double, 3>), grid, block, lds, st, A, B, C, batch, src_dist, dst_dist,
-                         scratch, slot_elems);
+      hipLaunchKernelGGL((fourstep_kernel<double, 3>), grid, block, lds, st, Q, in, out, scale, batch, src_dist,
+                         dst_dist, scratch, slot_elems);
   } else {
     if (nph == 2)
-      hipLaunchKernelGGL((fourstep_kernel<float, 2>), grid, block, lds, st, A, B, C, batch, src_dist, dst_dist,
-                         scratch, slot_elems);
+      hipLaunchKernelGGL((fourstep_kernel<float, 2>), grid, block, lds, st, Q, in, out, scale, batch, src_dist,
+                         dst_dist, scratch, slot_elems);
     else
-      hipLaunchKernelGGL((fourstep_kernel<float, 3>), grid, block, lds, st, A, B, C, batch, src_dist, dst_dist,
-                         scratch, slot_elems);
+      hipLaunchKernelGGL((fourstep_kernel<float, 3>), grid, block, lds, st, Q, in, out, scale, batch, src_dist,
+                         dst_dist, scratch, slot_elems);
   }
   return hipGetLastError();
 }

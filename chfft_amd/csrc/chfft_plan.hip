@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -34,7 +35,7 @@ hipError_t launch_fused(bool f64, int ld, int st, const KParams& P, dim3 grid, d
                         hipStream_t stream);
 hipError_t set_fused_lds_limits();
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
-hipError_t launch_fourstep(bool f64, int nph, const KParams& A, const KParams& B, const KParams& C, int batch,
+hipError_t launch_fourstep(bool f64, int nph, const KParams* Q, const void* in, void* out, double scale, int batch,
                            long long src_dist, long long dst_dist, void* scratch, long long slot_elems, dim3 grid,
                            dim3 block, size_t lds, hipStream_t stream);
 int fourstep_occupancy(bool f64, int nph, int block, size_t lds);
@@ -239,35 +240,43 @@ size_t lds_cap(bool f64) {
 
 // ------------------------------------------------------------------ launch geometry
 struct Geometry {
-  int tpb, Lp, block;
+  int tpb, tpt, Lp, block;
   size_t lds;
 };
 
-Geometry choose_geometry(int Lfft, long long G, bool colmajor, bool f64) {
-  const size_t cap = lds_cap(f64);
+// Workgroup shape for tpb transforms of length Lfft: tpt threads per
+// transform (>= Lfft / kEmax so one pass stages at most kEmax values per
+// thread; about 8 elements per thread otherwise), tpb a power of two
+// (16 for column-major access: 16 adjacent complex values per row segment),
+// block = tpb * tpt a whole number of wavefronts where possible.
+Geometry choose_geometry(int Lfft, long long G, bool colmajor, bool f64, int max_t = 0) {
+  const int maxT = max_t ? max_t : max_threads(f64);
   const size_t esz = esz_of(f64);
-  int tpb_max = 1;
-  if ((size_t)(Lfft + 1) * 2 <= cap) tpb_max = (int)(cap / (size_t)(Lfft + 1));
+  const int tpt_min = (Lfft + kEmax - 1) / kEmax;
+  int tpt_pref = std::max(tpt_min, std::min((Lfft + 7) / 8, 256));
+  auto fits = [&](int tpb, int tpt) {
+    int Lp = Lfft + (tpb > 1 ? 1 : 0);
+    return (long long)tpb * tpt <= maxT && (size_t)tpb * Lp * esz <= kLdsBytes;
+  };
   int tpb;
   if (colmajor) {
-    tpb = std::min(tpb_max, 16);
+    tpb = 16;
   } else {
-    tpb = std::max(1, 2048 / Lfft);
-    tpb = std::min(tpb, tpb_max);
+    tpb = 1;
+    while (tpb * 2 * Lfft <= 2048) tpb *= 2;
     while (tpb > 1 && (G + tpb - 1) / tpb < 1024) tpb >>= 1;
   }
-  tpb = std::max(tpb, 1);
+  while (tpb > 1 && !fits(tpb, tpt_min)) tpb >>= 1;
+  int tpt = tpt_pref;
+  const int gran = 64 / std::min(64, tpb);  // tpb is a power of two
+  tpt = (tpt + gran - 1) / gran * gran;
+  while (tpt > tpt_min && !fits(tpb, tpt)) tpt--;
+  if (!fits(tpb, tpt)) tpt = tpt_min;
   Geometry g;
   g.tpb = tpb;
+  g.tpt = tpt;
   g.Lp = Lfft + (tpb > 1 ? 1 : 0);
-  long long elems = (long long)tpb * Lfft;
-  int block = (int)((elems + 7) / 8);
-  block = (block + 63) / 64 * 64;
-  int minb = (int)((elems + kEmax - 1) / kEmax);
-  minb = (minb + 63) / 64 * 64;
-  block = std::max(block, minb);
-  block = std::max(64, std::min(max_threads(f64), block));
-  g.block = block;
+  g.block = tpb * tpt;
   g.lds = (size_t)tpb * g.Lp * esz;
   return g;
 }
@@ -285,6 +294,7 @@ struct Launch {
   int nph;
   long long src_dist, dst_dist, slot_elems;
   int batch;
+  void* qdev;  // device copy of Q[0..nph), owned by the plan
   dim3 grid, block;
   size_t lds;
   int src, dst;      // Slot
@@ -312,6 +322,7 @@ int build_seq(size_t L, bool f64, SeqDesc& seq, DevBuf& tw) {
     seq.pass[q].radix = rad[q];
     seq.pass[q].p = p;
     seq.pass[q].tws = (int)(L / ((size_t)p * rad[q]));
+    seq.pass[q].pmagic = (unsigned)((((unsigned long long)1 << 32) + (unsigned long long)p - 1) / (unsigned long long)p);
     p *= rad[q];
   }
   int rc = upload(roots_table(L), f64, tw);
@@ -616,8 +627,9 @@ struct C2CEngine {
       C.chirp = chirp.p;
       C.conj_out = ci;
     }
-    Geometry ga = choose_geometry((int)n1, 1 << 20, true, f64);
-    Geometry gb = choose_geometry((int)n2, 1 << 20, true, f64);
+    const int fs_max = f64 ? kMaxThreadsFsF64 : kMaxThreadsFsF32;
+    Geometry ga = choose_geometry((int)n1, 1 << 20, true, f64, fs_max);
+    Geometry gb = choose_geometry((int)n2, 1 << 20, true, f64, fs_max);
     A.tpb = ga.tpb;
     A.Lp = ga.Lp;
     B.tpb = gb.tpb;
@@ -683,6 +695,7 @@ struct chfft_plan_s {
   DevBuf tab_a, tab_b, tab_c;
   DevBuf scratch;
   DevBuf stage_in, stage_out;
+  std::vector<std::unique_ptr<DevBuf>> qbufs;  // four-step phase descriptors
   std::vector<Launch> fwd, bwd;
 };
 
@@ -724,6 +737,20 @@ int finish_plan(chfft_plan_s* p, chfft_plan* out) {
   if (p->kind == K_C2C2D) need = p->rows.scratch_elems(p->batch * p->m);
   need = std::max(need, p->rows.persistent_scratch_elems(p->fwd));
   need = std::max(need, p->rows.persistent_scratch_elems(p->bwd));
+  for (std::vector<Launch>* L : {&p->fwd, &p->bwd})
+    for (Launch& l : *L)
+      if (l.kind == 2) {
+        p->qbufs.emplace_back(new DevBuf());
+        DevBuf& q = *p->qbufs.back();
+        int rc2 = dev_alloc(q, sizeof(KParams) * l.nph);
+        if (!rc2 && hipMemcpy(q.p, l.Q, sizeof(KParams) * l.nph, hipMemcpyHostToDevice) != hipSuccess)
+          rc2 = CHFFT_ERR_HIP;
+        if (rc2) {
+          delete p;
+          return rc2;
+        }
+        l.qdev = q.p;
+      }
   int rc = dev_alloc(p->scratch, need * esz_of(p->f64));
   if (rc) {
     delete p;
@@ -1091,13 +1118,9 @@ int chfft_exec_range(chfft_plan plan, const void* in, void* out, int method, int
     const void* slots_src[3] = {in, out, plan->scratch.p};
     void* slots_dst[3] = {const_cast<void*>(in), out, plan->scratch.p};
     if (l0.kind == 2) {
-      KParams A = l0.Q[0], B = l0.Q[1], C = l0.Q[2];
-      A.src = slots_src[l0.src];
-      (l0.nph == 2 ? B : C).dst = slots_dst[l0.dst];
-      A.scale = B.scale = C.scale = 1.0;
-      (l0.nph == 2 ? B : C).scale = scale * l0.scale_mul;
-      if (launch_fourstep(plan->f64, l0.nph, A, B, C, l0.batch, l0.src_dist, l0.dst_dist, plan->scratch.p,
-                          l0.slot_elems, l0.grid, l0.block, l0.lds, st) != hipSuccess)
+      if (launch_fourstep(plan->f64, l0.nph, reinterpret_cast<const KParams*>(l0.qdev), slots_src[l0.src],
+                          slots_dst[l0.dst], scale * l0.scale_mul, l0.batch, l0.src_dist, l0.dst_dist,
+                          plan->scratch.p, l0.slot_elems, l0.grid, l0.block, l0.lds, st) != hipSuccess)
         return CHFFT_ERR_HIP;
       continue;
     }

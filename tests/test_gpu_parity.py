@@ -246,3 +246,20 @@ def test_cfft2d_config_size(dtype):
     y = f.forward(x)
     check(y, np.fft.fft2(x.astype(np.complex128)), f64, m * n)
     check(f.backward(y), x.astype(np.complex128), f64, m * n)
+
+
+@pytest.mark.parametrize("schedule", ["persistent", "split"])
+@pytest.mark.parametrize("n,batch", [(43200, 96), (10007, 64), (20000, 40), (3 ** 9, 33)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_fourstep_schedules(monkeypatch, schedule, n, batch, dtype):
+    # both four-step schedules (one persistent launch vs. split launches)
+    monkeypatch.setenv("CHFFT_FOURSTEP", schedule)
+    f64 = dtype == np.float64
+    cdt = np.complex128 if f64 else np.complex64
+    r = rng(n + batch)
+    x = (r.standard_normal((batch, n)) + 1j * r.standard_normal((batch, n))).astype(cdt)
+    fft = CFft1D.with_len(n, dtype)
+    xr = x.astype(np.complex128)
+    for m in ("forward", "backward", "forwardu"):
+        got = getattr(fft, m)(x)
+        check(got, oracle.c2c(xr, m), f64, n)
