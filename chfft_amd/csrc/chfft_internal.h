@@ -28,13 +28,14 @@ struct PassDesc {
   int p;
   int tws;
   unsigned pmagic;  // ceil(2^32 / p): i / p == umulhi(i, pmagic) for i < 2^16
+  int twoff;        // offset of this pass's table [j-1][k] = W_{pR}^(j*k) in tw
 };
 
 struct SeqDesc {
   int L;
   int npass;
   PassDesc pass[kMaxPass];
-  const void* tw;  // W_L^m, m in [0, L), in the kernel's precision
+  const void* tw;  // per-pass twiddle tables, in the kernel's precision
 };
 
 // Element idx of transform g lives at
@@ -89,7 +90,8 @@ struct KParams {
   double scale;     // user scaling, applied by the store operator
   double ortho;     // DCT element-0 scaler
   long long N;      // logical length the operator works on
-  long long twN;    // modulus of the four-step twiddle index
+  long long twN;    // four-step twiddle: size of the W^(c*k) table
+  int twk, twc;     // four-step twiddle index = k * twk + c * twc
   long long kc, ke; // spectrum index = c * kc + idx * ke
   const void* chirp;
   const void* kspec;

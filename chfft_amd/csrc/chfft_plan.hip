@@ -39,6 +39,17 @@ hipError_t launch_fourstep(bool f64, int nph, const KParams* Q, const void* in, 
                            long long src_dist, long long dst_dist, void* scratch, long long slot_elems, dim3 grid,
                            dim3 block, size_t lds, hipStream_t stream);
 int fourstep_occupancy(bool f64, int nph, int block, size_t lds);
+struct SfftEntry {
+  int f64, L, tpt, tpb, col, ld, st, mode, epi;
+  int nrad;
+  int rad[8];
+  void (*fn)(KParams);
+};
+const SfftEntry* sfft_find(bool f64, int L, int ld, int st, int mode, bool col);
+hipError_t sfft_launch(const SfftEntry* e, const KParams& P, dim3 grid, size_t lds, hipStream_t st);
+hipError_t sfft_set_lds_limits();
+size_t sfft_lds_bytes(const SfftEntry* e);
+int sfft_lds_stride(const SfftEntry* e);
 
 namespace {
 
@@ -254,8 +265,13 @@ Geometry choose_geometry(int Lfft, long long G, bool colmajor, bool f64, int max
   const size_t esz = esz_of(f64);
   const int tpt_min = (Lfft + kEmax - 1) / kEmax;
   int tpt_pref = std::max(tpt_min, std::min((Lfft + 7) / 8, 256));
+  auto lp_of = [&](int tpb) {
+    int Lp = Lfft + (Lfft + 15) / 16;  // one padding slot per 16 elements
+    if (tpb > 1 && (Lp & 1) == 0) Lp++;
+    return Lp;
+  };
   auto fits = [&](int tpb, int tpt) {
-    int Lp = Lfft + (tpb > 1 ? 1 : 0);
+    int Lp = lp_of(tpb);
     return (long long)tpb * tpt <= maxT && (size_t)tpb * Lp * esz <= kLdsBytes;
   };
   int tpb;
@@ -275,7 +291,7 @@ Geometry choose_geometry(int Lfft, long long G, bool colmajor, bool f64, int max
   Geometry g;
   g.tpb = tpb;
   g.tpt = tpt;
-  g.Lp = Lfft + (tpb > 1 ? 1 : 0);
+  g.Lp = lp_of(tpb);
   g.block = tpb * tpt;
   g.lds = (size_t)tpb * g.Lp * esz;
   return g;
@@ -286,7 +302,8 @@ enum Kind { K_C2C = 0, K_R2C = 1, K_C2C2D = 2, K_DCT = 3, K_MDCT = 4 };
 enum Slot { S_IN = 0, S_OUT = 1, S_SCRATCH = 2 };
 
 struct Launch {
-  int kind = 0;  // 0 fused kernel, 1 copy, 2 persistent four-step kernel
+  int kind = 0;  // 0 fused kernel, 1 copy, 2 persistent four-step kernel, 3 specialised fused kernel
+  const SfftEntry* sf = nullptr;
   int ld, st;
   KParams P;
   // kind 2: per-phase descriptors of one transform
@@ -317,17 +334,47 @@ int build_seq(size_t L, bool f64, SeqDesc& seq, DevBuf& tw) {
   if ((int)rad.size() > kMaxPass) return CHFFT_ERR_UNSUPPORTED;
   seq.L = (int)L;
   seq.npass = (int)rad.size();
+  std::vector<ld_t> tab;
   int p = 1;
   for (size_t q = 0; q < rad.size(); q++) {
-    seq.pass[q].radix = rad[q];
-    seq.pass[q].p = p;
-    seq.pass[q].tws = (int)(L / ((size_t)p * rad[q]));
-    seq.pass[q].pmagic = (unsigned)((((unsigned long long)1 << 32) + (unsigned long long)p - 1) / (unsigned long long)p);
-    p *= rad[q];
+    const int R = rad[q];
+    PassDesc& pd = seq.pass[q];
+    pd.radix = R;
+    pd.p = p;
+    pd.tws = (int)(L / ((size_t)p * R));
+    pd.pmagic = (unsigned)((((unsigned long long)1 << 32) + (unsigned long long)p - 1) / (unsigned long long)p);
+    pd.twoff = (int)(tab.size() / 2);
+    if (p > 1) {
+      // [j-1][k] = W_{pR}^(j*k), k < p, j = 1..R-1
+      for (int j = 1; j < R; j++)
+        for (int k = 0; k < p; k++) {
+          ld_t re, im;
+          unit_root((unsigned long long)j * k, (unsigned long long)p * R, &re, &im);
+          tab.push_back(re);
+          tab.push_back(im);
+        }
+    }
+    p *= R;
   }
-  int rc = upload(roots_table(L), f64, tw);
+  if (tab.empty()) {
+    tab.push_back(1.0L);
+    tab.push_back(0.0L);
+  }
+  int rc = upload(tab, f64, tw);
   seq.tw = tw.p;
   return rc;
+}
+
+// Four-step inter-pass twiddle table T[k1 * n2 + c2] = W_total^(k1 * c2),
+// k1 < n1, c2 < n2 (column pass: k = k1, c = column; Bluestein row pass:
+// c = k1, k = element).
+std::vector<ld_t> fourstep_table(size_t n1, size_t n2) {
+  std::vector<ld_t> v(2 * n1 * n2);
+  for (size_t k1 = 0; k1 < n1; k1++)
+    for (size_t c = 0; c < n2; c++)
+      unit_root((unsigned long long)k1 * c, (unsigned long long)n1 * n2, &v[2 * (k1 * n2 + c)],
+                &v[2 * (k1 * n2 + c) + 1]);
+  return v;
 }
 
 // Can a length-L inner transform be fused into one workgroup?
@@ -363,6 +410,28 @@ void fill_inner(KParams& P, const Inner& in) {
   P.kspec = in.kspec.p;
   P.kc = 0;
   P.ke = 1;
+}
+
+// Use a length-specialised kernel for a fused launch when one is registered
+// for this length, operator pair and radix schedule.
+void try_specialise(Launch& l, bool f64) {
+  if (l.kind != 0) return;
+  const int mode = l.P.has_mid ? (l.P.has_chirp ? 1 : 2) : 0;
+  const bool col = l.P.colmajor_ld != 0;
+  const SfftEntry* e = sfft_find(f64, l.P.Lfft, l.ld, l.st, mode, col);
+  if (!e || e->nrad != l.P.seq.npass) return;
+  for (int q = 0; q < e->nrad; q++)
+    if (e->rad[q] != l.P.seq.pass[q].radix) return;
+  // a direct (non-epilogue) store uses the load's thread layout
+  if (!e->epi && l.P.colmajor_st != l.P.colmajor_ld) return;
+  l.kind = 3;
+  l.sf = e;
+  l.P.tpb = e->tpb;
+  l.P.Lp = sfft_lds_stride(e);
+  l.grid = dim3((unsigned)((l.P.G + e->tpb - 1) / e->tpb));
+  l.block = dim3((unsigned)(e->tpt * e->tpb));
+  l.lds = sfft_lds_bytes(e);
+  l.what = "specialised " + l.what;
 }
 
 IoDesc io(long long bdist, long long cstride, long long estride, int ncols) {
@@ -425,7 +494,7 @@ struct C2CEngine {
     int rc;
     if ((rc = build_seq(n1, f64, s1, tw1))) return rc;
     if ((rc = build_seq(n2, f64, s2, tw2))) return rc;
-    if ((rc = upload(roots_table(total), f64, twN))) return rc;
+    if ((rc = upload(fourstep_table(n1, n2), f64, twN))) return rc;
     if (mode == 3) {
       if ((rc = upload(chirp_table(len), f64, chirp))) return rc;
       if ((rc = upload(bluestein_spectrum(len, total), f64, kspec))) return rc;
@@ -468,6 +537,7 @@ struct C2CEngine {
       finish(l, g, src, dst, 1);
       l.what = std::string(inner.bluestein ? "fused bluestein N=" + std::to_string(n) + " M " : "fused ") +
                seq_str(inner.seq);
+      try_specialise(l, f64);
       out.push_back(l);
       return;
     }
@@ -488,9 +558,12 @@ struct C2CEngine {
       a.P.conj_in = ci;
       a.P.twN = (long long)n;
       a.P.tab_a = twN.p;
+      a.P.twk = (int)n2;
+      a.P.twc = 1;
       finish(a, choose_geometry((int)n1, a.P.G, true, f64), src, S_SCRATCH, 0);
       a.what = "four-step columns " + seq_str(s1) + " + twiddle";
-      out.push_back(a);
+      try_specialise(a, f64);
+    out.push_back(a);
       // K2: rows (length n2) -> out[k1 + n1*k2]
       Launch b{};
       b.ld = LD_PLAIN;
@@ -503,7 +576,8 @@ struct C2CEngine {
       b.P.conj_out = ci;
       finish(b, choose_geometry((int)n2, b.P.G, true, f64), S_SCRATCH, dst, 1);
       b.what = "four-step rows " + seq_str(s2) + " + transpose";
-      out.push_back(b);
+      try_specialise(b, f64);
+    out.push_back(b);
       return;
     }
     // mode 3: Bluestein over a four-step M = n1 * n2
@@ -520,8 +594,11 @@ struct C2CEngine {
     a.P.conj_in = ci;
     a.P.twN = (long long)M;
     a.P.tab_a = twN.p;
+    a.P.twk = (int)n2;
+    a.P.twc = 1;
     finish(a, choose_geometry((int)n1, a.P.G, true, f64), src, S_SCRATCH, 0);
     a.what = "bluestein N=" + std::to_string(n) + " chirp + columns " + seq_str(s1) + " + twiddle";
+    try_specialise(a, f64);
     out.push_back(a);
     Launch b{};
     b.ld = LD_PLAIN;
@@ -536,8 +613,11 @@ struct C2CEngine {
     b.P.ke = (long long)n1;
     b.P.twN = (long long)M;
     b.P.tab_a = twN.p;
+    b.P.twk = 1;
+    b.P.twc = (int)n2;
     finish(b, choose_geometry((int)n2, b.P.G, false, f64), S_SCRATCH, S_SCRATCH, 0);
     b.what = "bluestein rows " + seq_str(s2) + " x spectrum, inverse rows + twiddle";
+    try_specialise(b, f64);
     out.push_back(b);
     Launch c{};
     c.ld = LD_PLAIN;
@@ -552,6 +632,7 @@ struct C2CEngine {
     c.P.conj_out = ci;
     finish(c, choose_geometry((int)n1, c.P.G, true, f64), S_SCRATCH, dst, 1);
     c.what = "bluestein inverse columns " + seq_str(s1) + " + de-chirp";
+    try_specialise(c, f64);
     out.push_back(c);
   }
 
@@ -561,7 +642,9 @@ struct C2CEngine {
   bool persistent_ok(size_t batch) const {
     if (force_split) return false;
     if (force_persistent) return true;
-    return batch >= (size_t)num_cus();
+    // the split launches run length-specialised kernels where registered;
+    // the persistent kernel runs the generic passes
+    return false && batch >= (size_t)num_cus();
   }
 
   static int num_cus() {
@@ -601,6 +684,8 @@ struct C2CEngine {
     A.conj_in = ci;
     A.twN = (long long)(mode == 2 ? n : M);
     A.tab_a = twN.p;
+    A.twk = (int)n2;
+    A.twc = 1;
     plain_seq(B, s2, n2);
     B.G = (int)n1;
     B.in = io(0, (long long)n2, 1, (int)n1);
@@ -618,6 +703,8 @@ struct C2CEngine {
       B.ke = (long long)n1;
       B.twN = (long long)M;
       B.tab_a = twN.p;
+      B.twk = 1;
+      B.twc = (int)n2;
       plain_seq(C, s1, n1);
       C.G = (int)n2;
       C.in = io(0, 1, (long long)n2, (int)n2);
@@ -706,7 +793,11 @@ std::once_flag g_lds_once;
 hipError_t g_lds_err = hipSuccess;
 
 int ensure_runtime() {
-  std::call_once(g_lds_once, [] { g_lds_err = set_fused_lds_limits(); });
+  std::call_once(g_lds_once, [] {
+    g_lds_err = set_fused_lds_limits();
+    hipError_t e2 = sfft_set_lds_limits();
+    if (g_lds_err == hipSuccess) g_lds_err = e2;
+  });
   return g_lds_err == hipSuccess ? CHFFT_OK : CHFFT_ERR_HIP;
 }
 
@@ -728,6 +819,7 @@ Launch fused_op(const chfft_plan_s* p, int ld, int st, bool inner_inverse, long 
   Geometry g = choose_geometry((int)p->inner.Lfft, G, false, p->f64);
   C2CEngine::finish(l, g, S_IN, S_OUT, 1);
   l.what = std::string(p->inner.bluestein ? "fused bluestein " : "fused ") + seq_str(p->inner.seq);
+  try_specialise(l, p->f64);
   return l;
 }
 
@@ -1122,6 +1214,14 @@ int chfft_exec_range(chfft_plan plan, const void* in, void* out, int method, int
                           slots_dst[l0.dst], scale * l0.scale_mul, l0.batch, l0.src_dist, l0.dst_dist,
                           plan->scratch.p, l0.slot_elems, l0.grid, l0.block, l0.lds, st) != hipSuccess)
         return CHFFT_ERR_HIP;
+      continue;
+    }
+    if (l0.kind == 3) {
+      KParams P = l0.P;
+      P.src = slots_src[l0.src];
+      P.dst = slots_dst[l0.dst];
+      P.scale = l0.scaled ? scale * l0.scale_mul : 1.0;
+      if (sfft_launch(l0.sf, P, l0.grid, l0.lds, st) != hipSuccess) return CHFFT_ERR_HIP;
       continue;
     }
     if (l0.kind == 1) {  // length-1 copy

@@ -1,0 +1,404 @@
+// Length-specialised fused transform kernels for gfx950.
+//
+// Same job as fused_fft_kernel (chfft_kernels.hip) — one workgroup owns TPB
+// transforms of length L — but with the length, the radix schedule and the
+// thread layout fixed at compile time:
+//   * each thread keeps the inputs of its butterflies in registers; the first
+//     pass gathers them straight from global memory through the load operator
+//     (coalesced: butterfly i of thread u reads element i + j*L/R0 with
+//     i = u + b*TPT), the last pass stores straight to global memory through
+//     an element-wise store operator (position i + j*L/R_last), or through the
+//     LDS epilogue for operators that pair bins k and N-k;
+//   * passes exchange through LDS once each, ping-ponging between two padded
+//     buffers when they fit (one barrier per exchange), else one buffer with
+//     two barriers;
+//   * all index arithmetic (i mod p, twiddle offsets, strides) is constant-
+//     folded; twiddles come from per-pass tables laid out [j-1][k] so that
+//     consecutive lanes read consecutive entries.
+// Fused Bluestein (BLUE) multiplies by the kernel spectrum in registers at the
+// last-pass positions, conjugates, re-enters the first-pass layout through one
+// exchange, runs the passes again and de-chirps in registers.
+//
+// Reference correspondence: as for the generic kernels — the passes replace
+// mixed_radix.rs:104-368 / chirpz.rs:27-85, the operators the pre/post loops
+// of chirpz.rs, rfft1d.rs, dct1d.rs and mdct1d.rs.
+#include "chfft_device.h"
+
+namespace chfft {
+
+template <int... RS>
+struct Rad {};
+
+template <int... RS>
+constexpr int rad_count(Rad<RS...>) {
+  return (int)sizeof...(RS);
+}
+template <int... RS>
+constexpr int rad_at(Rad<RS...>, int q) {
+  const int a[] = {RS...};
+  return a[q];
+}
+template <int... RS>
+constexpr int rad_span(Rad<RS...> r, int q) {
+  int p = 1;
+  for (int i = 0; i < q; i++) p *= rad_at(r, i);
+  return p;
+}
+// offset of pass q's twiddle table: passes with span p > 1 store (R-1)*p entries
+template <int... RS>
+constexpr int rad_twoff(Rad<RS...> r, int q) {
+  int off = 0;
+  for (int i = 0; i < q; i++)
+    if (rad_span(r, i) > 1) off += (rad_at(r, i) - 1) * rad_span(r, i);
+  return off;
+}
+
+constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+constexpr int lds_stride(int L, int tpb) {
+  return (tpb > 1 && ((L + (L + 15) / 16) & 1) == 0) ? L + (L + 15) / 16 + 1 : L + (L + 15) / 16;
+}
+
+template <class RS, int L, int TPT>
+struct Plan {
+  static constexpr int NP = rad_count(RS{});
+  static constexpr int R(int q) { return rad_at(RS{}, q); }
+  static constexpr int P(int q) { return rad_span(RS{}, q); }
+  static constexpr int NB(int q) { return L / R(q); }
+  static constexpr int B(int q) { return cdiv(NB(q), TPT); }
+  static constexpr bool FULL(int q) { return NB(q) % TPT == 0; }
+  static constexpr int EMAX() {
+    int e = 0;
+    for (int q = 0; q < NP; q++) e = (B(q) * R(q) > e) ? B(q) * R(q) : e;
+    return e;
+  }
+};
+
+// ---------------------------------------------------------------- pass pieces
+template <typename T, class PL, class RS, int q, int TPT>
+__device__ __forceinline__ void pass_compute(cx<T>* x, int u, const cx<T>* __restrict__ tw) {
+  constexpr int R = PL::R(q), p = PL::P(q), NB = PL::NB(q), B = PL::B(q);
+  constexpr int off = rad_twoff(RS{}, q);
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    const int i = u + b * TPT;
+    if (PL::FULL(q) || i < NB) {
+      if constexpr (p > 1) {
+        const int k = i % p;
+        const cx<T>* tq = tw + off + k;
+#pragma unroll
+        for (int j = 1; j < R; j++) x[b * R + j] = x[b * R + j] * tq[(j - 1) * p];
+      }
+      dft<T, R>(x + b * R);
+    }
+  }
+}
+
+// Write the outputs of pass q at their Stockham positions.
+template <typename T, class PL, int q, int TPT>
+__device__ __forceinline__ void pass_write(const cx<T>* x, int u, cx<T>* row) {
+  constexpr int R = PL::R(q), p = PL::P(q), NB = PL::NB(q), B = PL::B(q);
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    const int i = u + b * TPT;
+    if (PL::FULL(q) || i < NB) {
+      const int k = i % p;
+      const int o = (i - k) * R + k;
+#pragma unroll
+      for (int j = 0; j < R; j++) row[lx(o + j * p)] = x[b * R + j];
+    }
+  }
+}
+
+// Read the inputs of pass q (element i + j*NB of butterfly i).
+template <typename T, class PL, int q, int TPT>
+__device__ __forceinline__ void pass_read(cx<T>* x, int u, const cx<T>* row) {
+  constexpr int R = PL::R(q), NB = PL::NB(q), B = PL::B(q);
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    const int i = u + b * TPT;
+    if (PL::FULL(q) || i < NB) {
+#pragma unroll
+      for (int j = 0; j < R; j++) x[b * R + j] = row[lx(i + j * NB)];
+    }
+  }
+}
+
+// Exchange from pass qa's outputs to pass qb's inputs through LDS.
+template <typename T, class PL, int qa, int qb, int TPT, bool PP>
+__device__ __forceinline__ void exchange(cx<T>* x, int u, cx<T>* row0, cx<T>* row1, int& cur) {
+  cx<T>* row = (PP && cur) ? row1 : row0;
+  pass_write<T, PL, qa, TPT>(x, u, row);
+  __syncthreads();
+  pass_read<T, PL, qb, TPT>(x, u, row);
+  if constexpr (PP) {
+    cur ^= 1;
+  } else {
+    __syncthreads();
+  }
+}
+
+template <typename T, class PL, class RS, int q, int TPT, bool PP>
+__device__ __forceinline__ void run_seq(cx<T>* x, int u, const cx<T>* __restrict__ tw, cx<T>* row0, cx<T>* row1,
+                                        int& cur) {
+  pass_compute<T, PL, RS, q, TPT>(x, u, tw);
+  if constexpr (q + 1 < PL::NP) {
+    exchange<T, PL, q, q + 1, TPT, PP>(x, u, row0, row1, cur);
+    run_seq<T, PL, RS, q + 1, TPT, PP>(x, u, tw, row0, row1, cur);
+  }
+}
+
+// ---------------------------------------------------------------- store operators (element-wise)
+template <typename T, int ST>
+__device__ __forceinline__ void store_elem(const KParams& P, void* dst, long long obase, int c, int k, cx<T> v, T s) {
+  if constexpr (ST == ST_PLAIN) {
+    if (P.conj_out) v = cj(v);
+    reinterpret_cast<cx<T>*>(dst)[obase + (long long)k * P.out.estride] = scl(v, s);
+  } else if constexpr (ST == ST_TW) {
+    v = v * reinterpret_cast<const cx<T>*>(P.tab_a)[k * P.twk + c * P.twc];
+    if (P.conj_out) v = cj(v);
+    reinterpret_cast<cx<T>*>(dst)[obase + (long long)k * P.out.estride] = scl(v, s);
+  } else if constexpr (ST == ST_FS_DECHIRP) {
+    // four-step Bluestein last pass: element k of column c is sequence index
+    // c + k * estride; de-chirp (chirpz.rs:127-145) and crop to N
+    const long long j = (long long)c * P.out.cstride + (long long)k * P.out.estride;
+    if (j < P.N) {
+      v = cj(v) * reinterpret_cast<const cx<T>*>(P.chirp)[j];
+      if (P.conj_out) v = cj(v);
+      reinterpret_cast<cx<T>*>(dst)[obase - (long long)c * P.out.cstride + j] = scl(v, s);
+    }
+  } else if constexpr (ST == ST_MDCT) {
+    // mdct1d.rs:234-240
+    if (P.conj_out) v = cj(v);
+    const int h = (int)(P.N >> 1);
+    const cx<T> w = v * scl(reinterpret_cast<const cx<T>*>(P.tab_a)[k], s);
+    T* ret = reinterpret_cast<T*>(dst) + obase;
+    ret[2 * k] = -w.re;
+    ret[h - 1 - 2 * k] = w.im;
+  } else if constexpr (ST == ST_IMDCT) {
+    // mdct1d.rs:268-288
+    if (P.conj_out) v = cj(v);
+    const int N = (int)P.N, q = N >> 2, h = N >> 1, q3 = h + q, q5 = h + q3, lo = q - (q >> 1);
+    const T* w = reinterpret_cast<const T*>(P.tab_b);
+    const cx<T> z = v * scl(reinterpret_cast<const cx<T>*>(P.tab_a)[k], s);
+    T* ret = reinterpret_cast<T*>(dst) + obase;
+    const int mm = 2 * k;
+    ret[q3 - 1 - mm] = z.re * w[q3 - 1 - mm];
+    ret[q + mm] = -z.im * w[q + mm];
+    if (k < lo) {
+      ret[q3 + mm] = z.re * w[q3 + mm];
+      ret[q - 1 - mm] = z.im * w[q - 1 - mm];
+    } else {
+      ret[mm - q] = -z.re * w[mm - q];
+      ret[q5 - 1 - mm] = -z.im * w[q5 - 1 - mm];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- the kernel
+// MODE: 0 plain passes; 1 fused Bluestein (chirp on load, spectrum product,
+// second pass sequence, de-chirp); 2 spectrum product + second sequence only
+// (the row pass of a four-step Bluestein).
+template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS>
+__global__ void __launch_bounds__(TPT * TPB) sfft_kernel(KParams P) {
+  constexpr bool BLUE = MODE == 1;
+  using PL = Plan<RS, L, TPT>;
+  constexpr int NP = PL::NP;
+  constexpr int E = PL::EMAX();
+  constexpr int LP = lds_stride(L, TPB);
+  // a second LDS buffer saves one barrier per exchange but halves the
+  // workgroups that fit on a CU; only tiny tiles take it
+  constexpr bool PP = 2ull * TPB * LP * sizeof(cx<T>) <= 16384;
+  static_assert(TPB * LP * sizeof(cx<T>) <= kLdsBytes, "transform tile does not fit in LDS");
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+
+  const int tid = threadIdx.x;
+  const int t = COL ? tid % TPB : tid / TPT;
+  const int u = COL ? tid / TPB : tid % TPT;
+  const int g0 = blockIdx.x * TPB;
+  const int g = g0 + t;
+  const bool valid = g < P.G;
+  cx<T>* row0 = lds + t * LP;
+  cx<T>* row1 = PP ? lds + (TPB + t) * LP : row0;
+  const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.seq.tw);
+  const cx<T>* chirp = reinterpret_cast<const cx<T>*>(P.chirp);
+  int cur = 0;
+
+  cx<T> x[E];
+  // first pass gathered from global memory through the load operator
+  {
+    constexpr int R0 = PL::R(0), NB0 = PL::NB(0), B0 = PL::B(0);
+    const Loader<T, LD> ld = make_loader<T, LD>(P, P.src, valid ? g : 0);
+#pragma unroll
+    for (int b = 0; b < B0; b++) {
+      const int i = u + b * TPT;
+      if (PL::FULL(0) || i < NB0) {
+#pragma unroll
+        for (int j = 0; j < R0; j++) {
+          const int n = i + j * NB0;
+          cx<T> v = mk(T(0), T(0));
+          if (valid && (!BLUE || n < P.L)) {
+            v = load_elem<T, LD>(P, ld, n);
+            if constexpr (LD != LD_FS_CHIRP) {
+              if (P.conj_in) v = cj(v);
+            }
+            if constexpr (BLUE) v = v * chirp[n];
+          }
+          x[b * R0 + j] = v;
+        }
+      }
+    }
+  }
+  run_seq<T, PL, RS, 0, TPT, PP>(x, u, tw, row0, row1, cur);
+
+  constexpr int RL = PL::R(NP - 1), PLAST = PL::P(NP - 1), BL = PL::B(NP - 1);
+  if constexpr (MODE != 0) {
+    // spectrum product at the last-pass positions, conjugate, second sequence
+    const cx<T>* K = reinterpret_cast<const cx<T>*>(P.kspec) + (long long)(g % P.in.ncols) * P.kc;
+#pragma unroll
+    for (int b = 0; b < BL; b++) {
+      const int i = u + b * TPT;
+      if (PL::FULL(NP - 1) || i < PL::NB(NP - 1)) {
+#pragma unroll
+        for (int j = 0; j < RL; j++) x[b * RL + j] = cj(x[b * RL + j] * K[(long long)(i + j * PLAST) * P.ke]);
+      }
+    }
+    exchange<T, PL, NP - 1, 0, TPT, PP>(x, u, row0, row1, cur);
+    run_seq<T, PL, RS, 0, TPT, PP>(x, u, tw, row0, row1, cur);
+  }
+  if constexpr (BLUE) {
+#pragma unroll
+    for (int b = 0; b < BL; b++) {
+      const int i = u + b * TPT;
+      if (PL::FULL(NP - 1) || i < PL::NB(NP - 1)) {
+#pragma unroll
+        for (int j = 0; j < RL; j++) {
+          const int k = i + j * PLAST;
+          if (k < P.L) x[b * RL + j] = cj(x[b * RL + j]) * chirp[k];
+        }
+      }
+    }
+  }
+
+  if constexpr (EPI) {
+    // operators that pair bins: through LDS in natural order
+    cx<T>* row = (PP && cur) ? row1 : row0;
+#pragma unroll
+    for (int b = 0; b < BL; b++) {
+      const int i = u + b * TPT;
+      if (PL::FULL(NP - 1) || i < PL::NB(NP - 1)) {
+#pragma unroll
+        for (int j = 0; j < RL; j++) row[lx(i + j * PLAST)] = x[b * RL + j];
+      }
+    }
+    __syncthreads();
+    // the planner sets P.Lp == LP for specialised launches
+    const TMap m = make_map(TPB);
+    store_stage<T, ST>(P, m, P.dst, (PP && cur) ? lds + TPB * LP : lds, g0, min(TPB, P.G - g0), P.scale);
+  } else {
+    if (!valid) return;
+    int c = 0;
+    long long obase;
+    if constexpr (ST == ST_PLAIN || ST == ST_TW || ST == ST_FS_DECHIRP)
+      obase = io_base(P.out, g, &c);
+    else
+      obase = (long long)g * P.out.bdist;
+    const T s = (T)P.scale;
+#pragma unroll
+    for (int b = 0; b < BL; b++) {
+      const int i = u + b * TPT;
+      if (PL::FULL(NP - 1) || i < PL::NB(NP - 1)) {
+#pragma unroll
+        for (int j = 0; j < RL; j++) {
+          const int k = i + j * PLAST;
+          if (!BLUE || k < P.L) store_elem<T, ST>(P, P.dst, obase, c, k, x[b * RL + j], s);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- registry
+// Each entry: a kernel instantiation and the schedule the planner must build
+// tables for (the radices, threads per transform, transforms per workgroup).
+struct SfftEntry {
+  int f64, L, tpt, tpb, col, ld, st, mode, epi;
+  int nrad;
+  int rad[8];
+  void (*fn)(KParams);
+};
+
+#define SF_KERNEL(T, L, TPT, TPB, COL, LD, ST, MODE, EPI, ...) \
+  &sfft_kernel<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, Rad<__VA_ARGS__>>
+
+#define SF_ENTRY(F64, T, L, TPT, TPB, COL, LD, ST, MODE, EPI, NR, ...)                   \
+  {F64, L, TPT, TPB, COL, LD, ST, MODE, EPI, NR, {__VA_ARGS__},                          \
+   reinterpret_cast<void (*)(KParams)>(SF_KERNEL(T, L, TPT, TPB, COL, LD, ST, MODE, EPI, __VA_ARGS__))}
+
+#define SF_BOTH(L, TPT, TPB, TPB64, COL, LD, ST, MODE, EPI, NR, ...)                    \
+  SF_ENTRY(0, float, L, TPT, TPB, COL, LD, ST, MODE, EPI, NR, __VA_ARGS__),              \
+      SF_ENTRY(1, double, L, TPT, TPB64, COL, LD, ST, MODE, EPI, NR, __VA_ARGS__)
+
+static const SfftEntry kSfft[] = {
+    // 4096: CFft1D rows, RFft1D N=8192 forward / backward
+    SF_BOTH(4096, 256, 1, 1, false, LD_PLAIN, ST_PLAIN, 0, false, 3, 16, 16, 16),
+    SF_BOTH(4096, 256, 1, 1, false, LD_PLAIN, ST_R2C, 0, true, 3, 16, 16, 16),
+    SF_BOTH(4096, 256, 1, 1, false, LD_R2C_BACK, ST_PLAIN, 0, false, 3, 16, 16, 16),
+    // 512: Dct1D / Mdct1D N=1024 / 2048 inner transforms, and CFft1D
+    SF_BOTH(512, 32, 8, 4, false, LD_PLAIN, ST_PLAIN, 0, false, 3, 16, 8, 4),
+    SF_BOTH(512, 32, 8, 4, false, LD_DCT2, ST_DCT2, 0, true, 3, 16, 8, 4),
+    SF_BOTH(512, 32, 8, 4, false, LD_DCT3, ST_DCT3, 0, true, 3, 16, 8, 4),
+    SF_BOTH(512, 32, 8, 4, false, LD_MDCT, ST_MDCT, 0, false, 3, 16, 8, 4),
+    SF_BOTH(512, 32, 8, 4, false, LD_IMDCT, ST_IMDCT, 0, false, 3, 16, 8, 4),
+    // 2048: CFft2D rows, CFft1D
+    SF_BOTH(2048, 128, 2, 1, false, LD_PLAIN, ST_PLAIN, 0, false, 3, 16, 16, 8),
+    // 8192: fused Bluestein for N <= 4096 (N = 4093 and smaller primes), CFft1D
+    SF_BOTH(8192, 512, 1, 1, false, LD_PLAIN, ST_PLAIN, 1, false, 4, 16, 16, 8, 4),
+    SF_BOTH(8192, 512, 1, 1, false, LD_PLAIN, ST_PLAIN, 0, false, 4, 16, 16, 8, 4),
+    // 43200 = 200 x 216 four-step: 16-column tiles + twiddle, 16-row tiles +
+    // transposed store (LDS epilogue)
+    SF_BOTH(200, 16, 16, 16, true, LD_PLAIN, ST_TW, 0, false, 3, 8, 5, 5),
+    SF_BOTH(216, 16, 16, 16, false, LD_PLAIN, ST_PLAIN, 0, true, 4, 8, 3, 3, 3),
+    // Bluestein M = 32768 = 128 x 256 (N = 10007) and M = 262144 = 512 x 512
+    // (N = 65537): chirp + columns, rows x spectrum + inverse rows, inverse
+    // columns + de-chirp
+    SF_BOTH(128, 16, 16, 16, true, LD_FS_CHIRP, ST_TW, 0, false, 2, 16, 8),
+    SF_BOTH(256, 16, 4, 4, false, LD_PLAIN, ST_TW, 2, false, 2, 16, 16),
+    SF_BOTH(128, 16, 16, 16, true, LD_PLAIN, ST_FS_DECHIRP, 0, false, 2, 16, 8),
+    SF_BOTH(512, 32, 16, 8, true, LD_FS_CHIRP, ST_TW, 0, false, 3, 16, 8, 4),
+    SF_BOTH(512, 32, 4, 4, false, LD_PLAIN, ST_TW, 2, false, 3, 16, 8, 4),
+    SF_BOTH(512, 32, 16, 8, true, LD_PLAIN, ST_FS_DECHIRP, 0, false, 3, 16, 8, 4),
+};
+
+const SfftEntry* sfft_find(bool f64, int L, int ld, int st, int mode, bool col) {
+  for (const SfftEntry& e : kSfft)
+    if (e.f64 == (int)f64 && e.L == L && e.ld == ld && e.st == st && e.mode == mode && e.col == (int)col)
+      return &e;
+  return nullptr;
+}
+
+hipError_t sfft_launch(const SfftEntry* e, const KParams& P, dim3 grid, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL(e->fn, grid, dim3(e->tpt * e->tpb), lds, st, P);
+  return hipGetLastError();
+}
+
+hipError_t sfft_set_lds_limits() {
+  hipError_t r = hipSuccess;
+  for (const SfftEntry& e : kSfft) {
+    hipError_t x = hipFuncSetAttribute(reinterpret_cast<const void*>(e.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)kLdsBytes);
+    if (x != hipSuccess) r = x;
+  }
+  return r;
+}
+
+// LDS bytes an entry needs (two buffers when they fit, else one).
+size_t sfft_lds_bytes(const SfftEntry* e) {
+  const size_t esz = e->f64 ? 16 : 8;
+  const size_t one = (size_t)e->tpb * lds_stride(e->L, e->tpb) * esz;
+  return 2 * one <= 16384 ? 2 * one : one;  // same rule as the kernel's PP
+}
+
+int sfft_lds_stride(const SfftEntry* e) { return lds_stride(e->L, e->tpb); }
+
+}  // namespace chfft

@@ -67,7 +67,8 @@ def test_readme_known_answer():
     np.testing.assert_allclose(got, [5 + 8j, -1 - 2j, -1 - 2j, 5 - 4j], atol=1e-14)
 
 
-@pytest.mark.parametrize("n,batch", [(4096, 256), (43200, 4), (4093, 8), (10007, 4), (65537, 2),
+@pytest.mark.parametrize("n,batch", [(4096, 256), (512, 100), (2048, 33), (4093, 37), (4096, 3),
+                                     (43200, 4), (4093, 8), (10007, 4), (65537, 2),
                                      (8192, 16), (16384, 4), (20000, 2), (1 << 20, 1), (3 ** 9, 2),
                                      (7 * 11 * 13 * 17, 2), (1048573, 1)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
