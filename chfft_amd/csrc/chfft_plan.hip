@@ -50,6 +50,13 @@ hipError_t sfft_launch(const SfftEntry* e, const KParams& P, dim3 grid, size_t l
 hipError_t sfft_set_lds_limits();
 size_t sfft_lds_bytes(const SfftEntry* e);
 int sfft_lds_stride(const SfftEntry* e);
+struct SfsEntry;
+const SfsEntry* sfs_find(bool f64, int n1, int n2, int nph);
+void sfs_info(const SfsEntry* e, int* threads, int* tpb, int* Lp, size_t* lds);
+hipError_t sfs_launch(const SfsEntry* e, const KParams* Q, const void* in, void* out, double scale, int batch,
+                      long long src_dist, long long dst_dist, void* scratch, long long slot_elems, dim3 grid,
+                      hipStream_t st);
+int sfs_occupancy(const SfsEntry* e);
 
 namespace {
 
@@ -302,8 +309,10 @@ enum Kind { K_C2C = 0, K_R2C = 1, K_C2C2D = 2, K_DCT = 3, K_MDCT = 4 };
 enum Slot { S_IN = 0, S_OUT = 1, S_SCRATCH = 2 };
 
 struct Launch {
-  int kind = 0;  // 0 fused kernel, 1 copy, 2 persistent four-step kernel, 3 specialised fused kernel
+  int kind = 0;  // 0 fused kernel, 1 copy, 2 persistent four-step kernel, 3 specialised fused kernel,
+                 // 4 specialised persistent four-step kernel
   const SfftEntry* sf = nullptr;
+  const SfsEntry* sfs = nullptr;
   int ld, st;
   KParams P;
   // kind 2: per-phase descriptors of one transform
@@ -484,9 +493,18 @@ struct C2CEngine {
     M = total;
     // most balanced split total = n1 * n2 with n1 <= n2, both fusable with
     // several transforms per workgroup
+    // split total = n1 * n2 (n1 <= n2): rows of the column pass 128-byte
+    // aligned (n2 a multiple of 16) if possible, then the most balanced
     size_t best1 = 0;
-    for (size_t d = 1; d * d <= total; d++)
-      if (total % d == 0 && smooth7(d) && smooth7(total / d)) best1 = d;
+    bool best_aligned = false;
+    for (size_t d = 1; d * d <= total; d++) {
+      if (total % d != 0 || !smooth7(d) || !smooth7(total / d)) continue;
+      const bool aligned = (total / d) % 16 == 0;
+      if (aligned || !best_aligned) {
+        best1 = d;
+        best_aligned = aligned;
+      }
+    }
     if (best1 == 0) return CHFFT_ERR_UNSUPPORTED;
     n1 = best1;
     n2 = total / best1;
@@ -642,9 +660,14 @@ struct C2CEngine {
   bool persistent_ok(size_t batch) const {
     if (force_split) return false;
     if (force_persistent) return true;
-    // the split launches run length-specialised kernels where registered;
-    // the persistent kernel runs the generic passes
-    return false && batch >= (size_t)num_cus();
+    // the persistent launch needs a registered specialised instantiation
+    // (the generic persistent kernel is kept for forced use only)
+    // Measured on MI355X (43200 x 512 FP32): the persistent launch, one
+    // transform per workgroup, ran at 449 us against 258 us for the split
+    // launches (its tiles run back to back with little memory-level
+    // parallelism), so it is only used when requested.
+    (void)batch;
+    return false;
   }
 
   static int num_cus() {
@@ -714,6 +737,35 @@ struct C2CEngine {
       C.chirp = chirp.p;
       C.conj_out = ci;
     }
+    if (const SfsEntry* e = sfs_find(f64, (int)n1, (int)n2, l.nph)) {
+      int threads, tpb[3], Lp[3];
+      size_t lds;
+      sfs_info(e, &threads, tpb, Lp, &lds);
+      A.tpb = tpb[0];
+      A.Lp = Lp[0];
+      B.tpb = tpb[1];
+      B.Lp = Lp[1];
+      C.tpb = tpb[2];
+      C.Lp = Lp[2];
+      l.kind = 4;
+      l.sfs = e;
+      l.block = dim3((unsigned)threads);
+      l.lds = lds;
+      l.batch = (int)batch;
+      l.src_dist = dist_in;
+      l.dst_dist = dist_out;
+      l.slot_elems = (long long)(mode == 2 ? n : M);
+      long long gsz = (long long)sfs_occupancy(e) * num_cus();
+      l.grid = dim3((unsigned)std::max(1LL, std::min(gsz, (long long)batch)));
+      l.src = src;
+      l.dst = dst;
+      l.scaled = 1;
+      l.scale_mul = 1.0;
+      l.what = std::string("specialised persistent four-step ") + (mode == 3 ? "bluestein " : "") + "columns " +
+               seq_str(s1) + " x rows " + seq_str(s2);
+      out.push_back(l);
+      return;
+    }
     const int fs_max = f64 ? kMaxThreadsFsF64 : kMaxThreadsFsF32;
     Geometry ga = choose_geometry((int)n1, 1 << 20, true, f64, fs_max);
     Geometry gb = choose_geometry((int)n2, 1 << 20, true, f64, fs_max);
@@ -744,7 +796,7 @@ struct C2CEngine {
   size_t persistent_scratch_elems(const std::vector<Launch>& L) const {
     size_t need = 0;
     for (const Launch& l : L)
-      if (l.kind == 2) need = std::max(need, (size_t)l.grid.x * (size_t)l.slot_elems);
+      if (l.kind == 2 || l.kind == 4) need = std::max(need, (size_t)l.grid.x * (size_t)l.slot_elems);
     return need;
   }
 
@@ -831,7 +883,7 @@ int finish_plan(chfft_plan_s* p, chfft_plan* out) {
   need = std::max(need, p->rows.persistent_scratch_elems(p->bwd));
   for (std::vector<Launch>* L : {&p->fwd, &p->bwd})
     for (Launch& l : *L)
-      if (l.kind == 2) {
+      if (l.kind == 2 || l.kind == 4) {
         p->qbufs.emplace_back(new DevBuf());
         DevBuf& q = *p->qbufs.back();
         int rc2 = dev_alloc(q, sizeof(KParams) * l.nph);
@@ -1213,6 +1265,13 @@ int chfft_exec_range(chfft_plan plan, const void* in, void* out, int method, int
       if (launch_fourstep(plan->f64, l0.nph, reinterpret_cast<const KParams*>(l0.qdev), slots_src[l0.src],
                           slots_dst[l0.dst], scale * l0.scale_mul, l0.batch, l0.src_dist, l0.dst_dist,
                           plan->scratch.p, l0.slot_elems, l0.grid, l0.block, l0.lds, st) != hipSuccess)
+        return CHFFT_ERR_HIP;
+      continue;
+    }
+    if (l0.kind == 4) {
+      if (sfs_launch(l0.sfs, reinterpret_cast<const KParams*>(l0.qdev), slots_src[l0.src], slots_dst[l0.dst],
+                     scale * l0.scale_mul, l0.batch, l0.src_dist, l0.dst_dist, plan->scratch.p, l0.slot_elems,
+                     l0.grid, st) != hipSuccess)
         return CHFFT_ERR_HIP;
       continue;
     }

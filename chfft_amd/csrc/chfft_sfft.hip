@@ -26,6 +26,8 @@
 
 namespace chfft {
 
+#define COMMA ,
+
 template <int... RS>
 struct Rad {};
 
@@ -199,7 +201,8 @@ __device__ __forceinline__ void store_elem(const KParams& P, void* dst, long lon
 // second pass sequence, de-chirp); 2 spectrum product + second sequence only
 // (the row pass of a four-step Bluestein).
 template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS>
-__global__ void __launch_bounds__(TPT * TPB) sfft_kernel(KParams P) {
+__device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, void* dst, int g0, double scale,
+                                          cx<T>* lds) {
   constexpr bool BLUE = MODE == 1;
   using PL = Plan<RS, L, TPT>;
   constexpr int NP = PL::NP;
@@ -209,13 +212,9 @@ __global__ void __launch_bounds__(TPT * TPB) sfft_kernel(KParams P) {
   // workgroups that fit on a CU; only tiny tiles take it
   constexpr bool PP = 2ull * TPB * LP * sizeof(cx<T>) <= 16384;
   static_assert(TPB * LP * sizeof(cx<T>) <= kLdsBytes, "transform tile does not fit in LDS");
-  extern __shared__ __align__(16) unsigned char smem[];
-  cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
-
   const int tid = threadIdx.x;
   const int t = COL ? tid % TPB : tid / TPT;
   const int u = COL ? tid / TPB : tid % TPT;
-  const int g0 = blockIdx.x * TPB;
   const int g = g0 + t;
   const bool valid = g < P.G;
   cx<T>* row0 = lds + t * LP;
@@ -228,7 +227,7 @@ __global__ void __launch_bounds__(TPT * TPB) sfft_kernel(KParams P) {
   // first pass gathered from global memory through the load operator
   {
     constexpr int R0 = PL::R(0), NB0 = PL::NB(0), B0 = PL::B(0);
-    const Loader<T, LD> ld = make_loader<T, LD>(P, P.src, valid ? g : 0);
+    const Loader<T, LD> ld = make_loader<T, LD>(P, src, valid ? g : 0);
 #pragma unroll
     for (int b = 0; b < B0; b++) {
       const int i = u + b * TPT;
@@ -294,7 +293,7 @@ __global__ void __launch_bounds__(TPT * TPB) sfft_kernel(KParams P) {
     __syncthreads();
     // the planner sets P.Lp == LP for specialised launches
     const TMap m = make_map(TPB);
-    store_stage<T, ST>(P, m, P.dst, (PP && cur) ? lds + TPB * LP : lds, g0, min(TPB, P.G - g0), P.scale);
+    store_stage<T, ST>(P, m, dst, (PP && cur) ? lds + TPB * LP : lds, g0, min(TPB, P.G - g0), scale);
   } else {
     if (!valid) return;
     int c = 0;
@@ -303,7 +302,7 @@ __global__ void __launch_bounds__(TPT * TPB) sfft_kernel(KParams P) {
       obase = io_base(P.out, g, &c);
     else
       obase = (long long)g * P.out.bdist;
-    const T s = (T)P.scale;
+    const T s = (T)scale;
 #pragma unroll
     for (int b = 0; b < BL; b++) {
       const int i = u + b * TPT;
@@ -311,12 +310,89 @@ __global__ void __launch_bounds__(TPT * TPB) sfft_kernel(KParams P) {
 #pragma unroll
         for (int j = 0; j < RL; j++) {
           const int k = i + j * PLAST;
-          if (!BLUE || k < P.L) store_elem<T, ST>(P, P.dst, obase, c, k, x[b * RL + j], s);
+          if (!BLUE || k < P.L) store_elem<T, ST>(P, dst, obase, c, k, x[b * RL + j], s);
         }
       }
     }
   }
 }
+
+// Tile index of this workgroup, arranged so that the workgroups one XCD runs
+// (hardware assigns workgroup b to XCD b mod 8) take consecutive tiles: tiles
+// that share cache lines at their edges then meet in the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  constexpr int X = 8;
+  const int x = b % X, q = nb / X, r = nb % X;
+  const int first = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+  return first + b / X;
+}
+
+template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS>
+__global__ void __launch_bounds__(TPT * TPB) sfft_kernel(KParams P) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  sfft_tile<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, RS>(P, P.src, P.dst, tile * TPB, P.scale, lds);
+}
+
+// Persistent four-step transform with length-specialised phases: one
+// transform per workgroup iteration, columns (A) then rows (B) [then inverse
+// columns (C) for Bluestein] through the workgroup's own scratch slot, which
+// is written and re-read within the same workgroup (L2/MALL resident).
+// Phase descriptors live in device memory (Q); src/dst advance by
+// src_dist/dst_dist elements per transform.
+template <typename T, class A, class B, class C>
+__global__ void __launch_bounds__(A::THREADS, 4) sfs_kernel(const KParams* __restrict__ Q, const void* in, void* out,
+                                                        double scale, int batch, long long src_dist,
+                                                        long long dst_dist, void* scratch, long long slot_elems) {
+  static_assert(A::THREADS == B::THREADS && A::THREADS == C::THREADS, "phases share the workgroup");
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* slot = reinterpret_cast<cx<T>*>(scratch) + (long long)blockIdx.x * slot_elems;
+  for (int tr = blockIdx.x; tr < batch; tr += gridDim.x) {
+    const cx<T>* src = reinterpret_cast<const cx<T>*>(in) + (long long)tr * src_dist;
+    cx<T>* dst = reinterpret_cast<cx<T>*>(out) + (long long)tr * dst_dist;
+    const KParams& QA = Q[0];
+    const KParams& QB = Q[1];
+    constexpr bool three = C::PHASE;
+    for (int g0 = 0; g0 < QA.G; g0 += A::TPB_) {
+      A::template run<T>(QA, src, slot, g0, 1.0, lds);
+      __syncthreads();
+    }
+    for (int g0 = 0; g0 < QB.G; g0 += B::TPB_) {
+      B::template run<T>(QB, slot, three ? (void*)slot : (void*)dst, g0, three ? 1.0 : scale, lds);
+      __syncthreads();
+    }
+    if constexpr (three) {
+      const KParams& QC = Q[2];
+      for (int g0 = 0; g0 < QC.G; g0 += C::TPB_) {
+        C::template run<T>(QC, slot, dst, g0, scale, lds);
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// A phase of sfs_kernel: one sfft_tile configuration.
+template <int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS>
+struct Phase {
+  static constexpr int THREADS = TPT * TPB;
+  static constexpr int TPB_ = TPB;
+  static constexpr bool PHASE = true;
+  template <typename T>
+  __device__ __forceinline__ static void run(const KParams& P, const void* src, void* dst, int g0, double scale,
+                                             cx<T>* lds) {
+    sfft_tile<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, RS>(P, src, dst, g0, scale, lds);
+  }
+};
+template <int THREADS_>
+struct NoPhase {
+  static constexpr int THREADS = THREADS_;
+  static constexpr int TPB_ = 1;
+  static constexpr bool PHASE = false;
+  template <typename T>
+  __device__ __forceinline__ static void run(const KParams&, const void*, void*, int, double, cx<T>*) {}
+};
 
 // ---------------------------------------------------------------- registry
 // Each entry: a kernel instantiation and the schedule the planner must build
@@ -359,6 +435,9 @@ static const SfftEntry kSfft[] = {
     // transposed store (LDS epilogue)
     SF_BOTH(200, 16, 16, 16, true, LD_PLAIN, ST_TW, 0, false, 3, 8, 5, 5),
     SF_BOTH(216, 16, 16, 16, false, LD_PLAIN, ST_PLAIN, 0, true, 4, 8, 3, 3, 3),
+    // 43200 = 180 x 240 (rows of the column pass 128-byte aligned)
+    SF_BOTH(180, 16, 16, 8, true, LD_PLAIN, ST_TW, 0, false, 4, 4, 5, 3, 3),
+    SF_BOTH(240, 16, 16, 8, false, LD_PLAIN, ST_PLAIN, 0, true, 3, 16, 5, 3),
     // Bluestein M = 32768 = 128 x 256 (N = 10007) and M = 262144 = 512 x 512
     // (N = 65537): chirp + columns, rows x spectrum + inverse rows, inverse
     // columns + de-chirp
@@ -369,6 +448,72 @@ static const SfftEntry kSfft[] = {
     SF_BOTH(512, 32, 4, 4, false, LD_PLAIN, ST_TW, 2, false, 3, 16, 8, 4),
     SF_BOTH(512, 32, 16, 8, true, LD_PLAIN, ST_FS_DECHIRP, 0, false, 3, 16, 8, 4),
 };
+
+// Persistent four-step instantiations.
+struct SfsEntry {
+  int f64, n1, n2, nph;
+  int threads;
+  int tpb[3];
+  int Lp[3];
+  size_t lds;
+  const void* fn;
+};
+
+template <int L, int TPB, bool F64>
+constexpr size_t tile_bytes() {
+  return (size_t)TPB * lds_stride(L, TPB) * (F64 ? 16 : 8);
+}
+constexpr size_t mx(size_t a, size_t b) { return a > b ? a : b; }
+
+#define SFS2(F64, T, N1, N2, TPT, TPB, RA, RB)                                                                  \
+  {F64, N1, N2, 2, TPT * TPB, {TPB, TPB, 1}, {lds_stride(N1, TPB), lds_stride(N2, TPB), 0},                    \
+   mx(tile_bytes<N1, TPB, F64>(), tile_bytes<N2, TPB, F64>()),                                                  \
+   reinterpret_cast<const void*>(                                                                               \
+       &sfs_kernel<T, Phase<N1, TPT, TPB, true, LD_PLAIN, ST_TW, 0, false, RA>,                                \
+                   Phase<N2, TPT, TPB, false, LD_PLAIN, ST_PLAIN, 0, true, RB>, NoPhase<TPT * TPB>>)}
+
+#define SFS3(F64, T, N1, N2, TPT, TPB, RA, RB)                                                                  \
+  {F64, N1, N2, 3, TPT * TPB, {TPB, TPB, TPB}, {lds_stride(N1, TPB), lds_stride(N2, TPB), lds_stride(N1, TPB)}, \
+   mx(tile_bytes<N1, TPB, F64>(), tile_bytes<N2, TPB, F64>()),                                                  \
+   reinterpret_cast<const void*>(                                                                               \
+       &sfs_kernel<T, Phase<N1, TPT, TPB, true, LD_FS_CHIRP, ST_TW, 0, false, RA>,                             \
+                   Phase<N2, TPT, TPB, false, LD_PLAIN, ST_TW, 2, false, RB>,                                  \
+                   Phase<N1, TPT, TPB, true, LD_PLAIN, ST_FS_DECHIRP, 0, false, RA>>)}
+
+static const SfsEntry kSfs[] = {
+    SFS2(0, float, 180, 240, 16, 16, Rad<4 COMMA 5 COMMA 3 COMMA 3>, Rad<16 COMMA 5 COMMA 3>),
+    SFS2(1, double, 180, 240, 16, 8, Rad<4 COMMA 5 COMMA 3 COMMA 3>, Rad<16 COMMA 5 COMMA 3>),
+    SFS3(0, float, 128, 256, 16, 16, Rad<16 COMMA 8>, Rad<16 COMMA 16>),
+    SFS3(1, double, 128, 256, 16, 8, Rad<16 COMMA 8>, Rad<16 COMMA 16>),
+};
+
+const SfsEntry* sfs_find(bool f64, int n1, int n2, int nph) {
+  for (const SfsEntry& e : kSfs)
+    if (e.f64 == (int)f64 && e.n1 == n1 && e.n2 == n2 && e.nph == nph) return &e;
+  return nullptr;
+}
+
+void sfs_info(const SfsEntry* e, int* threads, int* tpb, int* Lp, size_t* lds) {
+  *threads = e->threads;
+  for (int i = 0; i < 3; i++) {
+    tpb[i] = e->tpb[i];
+    Lp[i] = e->Lp[i];
+  }
+  *lds = e->lds;
+}
+
+hipError_t sfs_launch(const SfsEntry* e, const KParams* Q, const void* in, void* out, double scale, int batch,
+                      long long src_dist, long long dst_dist, void* scratch, long long slot_elems, dim3 grid,
+                      hipStream_t st) {
+  void* args[] = {&Q, &in, &out, &scale, &batch, &src_dist, &dst_dist, &scratch, &slot_elems};
+  return hipLaunchKernel(e->fn, grid, dim3(e->threads), args, e->lds, st);
+}
+
+int sfs_occupancy(const SfsEntry* e) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, e->fn, e->threads, e->lds) != hipSuccess) return 1;
+  return n > 0 ? n : 1;
+}
 
 const SfftEntry* sfft_find(bool f64, int L, int ld, int st, int mode, bool col) {
   for (const SfftEntry& e : kSfft)
@@ -384,6 +529,10 @@ hipError_t sfft_launch(const SfftEntry* e, const KParams& P, dim3 grid, size_t l
 
 hipError_t sfft_set_lds_limits() {
   hipError_t r = hipSuccess;
+  for (const SfsEntry& e : kSfs) {
+    hipError_t x = hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes);
+    if (x != hipSuccess) r = x;
+  }
   for (const SfftEntry& e : kSfft) {
     hipError_t x = hipFuncSetAttribute(reinterpret_cast<const void*>(e.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)kLdsBytes);
