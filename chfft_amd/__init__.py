@@ -5,5 +5,5 @@ libchfft_hip.so through the C ABI of include/chfft_hip.h.
 """
 from ._lib import build, lib, LIB_PATH, ChfftError  # noqa: F401
 from .api import (  # noqa: F401
-    CFft1D, CFft2D, Dct1D, DctType, Mdct1D, Plan, RFft1D, sine_window, vorbis_window,
+    CFft1D, CFft2D, Dct1D, DctType, Mdct1D, MultiFFT, Plan, RFft1D, sine_window, vorbis_window,
 )

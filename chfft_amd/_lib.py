@@ -8,7 +8,7 @@ import subprocess
 _PKG = os.path.dirname(os.path.abspath(__file__))
 _ROOT = os.path.dirname(_PKG)
 LIB_PATH = os.path.join(_PKG, "libchfft_hip.so")
-SOURCES = ["csrc/chfft_kernels.hip", "csrc/chfft_sfft.hip", "csrc/chfft_plan.hip"]
+SOURCES = ["csrc/chfft_kernels.hip", "csrc/chfft_sfft.hip", "csrc/chfft_tables.hip", "csrc/chfft_plan.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # no packed FP32 instructions (v_pk_*_f32) in any kernel
@@ -49,6 +49,9 @@ def _signatures():
         "chfft_plan_c2c_2d": ([pp, sz, sz, sz, ci], ci),
         "chfft_plan_dct_1d": ([pp, ci, sz, ci, sz, ci], ci),
         "chfft_plan_mdct_1d": ([pp, sz, vp, sz, sz, ci], ci),
+        "chfft_plan_c2c_multi": ([pp, vp, sz, ci], ci),
+        "chfft_plan_set_grid": ([vp, ci], ci),
+        "chfft_plan_autotune": ([vp, vp, vp, ci, vp], ci),
         "chfft_plan_destroy": ([vp], ci),
         "chfft_exec": ([vp, vp, vp, ci, vp], ci),
         "chfft_exec_host": ([vp, vp, vp, ci], ci),
@@ -65,6 +68,15 @@ def lib():
     """Load the built library (does not build it: build() first)."""
     global _LIB
     if _LIB is None:
+        # PyTorch-ROCm ships its own HIP runtime (torch/lib/libamdhip64.so,
+        # SONAME libamdhip64.so.7). Importing torch first makes this library's
+        # libamdhip64.so.7 dependency bind to that same copy, so that one
+        # process never holds two HIP runtimes (the second one to initialise
+        # would find no GPU).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"HIP extension not built: {LIB_PATH} is missing (run chfft_amd.build())")
         L = ctypes.CDLL(LIB_PATH)

@@ -28,7 +28,8 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ["CFft1D", "RFft1D", "CFft2D", "Dct1D", "DctType", "Mdct1D", "Plan", "sine_window", "vorbis_window"]
+__all__ = ["CFft1D", "RFft1D", "CFft2D", "Dct1D", "DctType", "Mdct1D", "MultiFFT", "Plan", "sine_window",
+           "vorbis_window"]
 
 
 def _is_torch(x):
@@ -69,6 +70,9 @@ class Plan:
         elif kind == "mdct":
             win = np.ascontiguousarray(args[1], dtype=np.float64)
             st = L.chfft_plan_mdct_1d(ctypes.byref(h), args[0], win.ctypes.data, batch, args[2], prec)
+        elif kind == "c2c_multi":
+            lens = np.ascontiguousarray(args[0], dtype=np.uint64)
+            st = L.chfft_plan_c2c_multi(ctypes.byref(h), lens.ctypes.data, len(lens), prec)
         else:
             raise ValueError(kind)
         _lib.check(st, f"plan {kind}{args}")
@@ -96,6 +100,18 @@ class Plan:
         buf = ctypes.create_string_buffer(8192)
         _lib.lib().chfft_plan_describe(self._h, buf, 8192)
         return buf.value.decode()
+
+    def autotune(self, src, dst, reps=3, stream=0):
+        """Time this plan's alternative schedules on device buffers (ints or
+        torch tensors; contents are overwritten) and keep the fastest."""
+        if _is_torch(src):
+            stream = stream or _torch().cuda.current_stream(src.device).cuda_stream
+            src, dst = src.data_ptr(), dst.data_ptr()
+        r = _lib.lib().chfft_plan_autotune(self._h, ctypes.c_void_p(src), ctypes.c_void_p(dst), reps,
+                                           ctypes.c_void_p(stream))
+        if r < 0:
+            _lib.check(-r, "autotune")
+        return r
 
     def exec_ptr(self, src, dst, method, stream=0):
         """Device pointers (ints); asynchronous on `stream` (hipStream_t as int)."""
@@ -519,3 +535,46 @@ class Mdct1D(_Base):
         if x.ndim != 1 or frames < 1:
             raise ValueError(f"invalid length (soure: {x.shape[-1]}, mdct.len: {self.len})")
         return self._run(x, tch, frames, hop, (frames, self.len // 2), method)
+
+
+class MultiFFT(_Base):
+    """Many complex transforms of uneven lengths in one persistent launch
+    (chfft_plan_c2c_multi). Each transform behaves as CFft1D of its own length
+    (cfft1d.rs:208-230, including the scalings per method); the signals are
+    packed back to back in one 1-D array of sum(lens) elements."""
+
+    def __init__(self, lens, dtype=np.float32):
+        super().__init__(dtype)
+        self.lens = [int(n) for n in lens]
+        self.total = sum(self.lens)
+        self.offsets = np.concatenate([[0], np.cumsum(self.lens)]).astype(np.int64)
+        self.plan = Plan("c2c_multi", self.f64, 1, self.lens)
+
+    def split(self, packed):
+        return [packed[self.offsets[i]:self.offsets[i + 1]] for i in range(len(self.lens))]
+
+    def _convert(self, packed, method):
+        x, tch = self._prep(packed, True)
+        if x.ndim != 1 or x.shape[0] != self.total:
+            raise ValueError(f"invalid length (soure: {x.shape[0]}, total: {self.total})")
+        out = self._empty(x, x.shape, True, tch)
+        self.plan.exec(x, out, method)
+        return out
+
+    def forward(self, packed):
+        return self._convert(packed, "forward")
+
+    def forwardu(self, packed):
+        return self._convert(packed, "forwardu")
+
+    def forwardn(self, packed):
+        return self._convert(packed, "forwardn")
+
+    def backward(self, packed):
+        return self._convert(packed, "backward")
+
+    def backward0(self, packed):
+        return self._convert(packed, "backward0")
+
+    def backwardu(self, packed):
+        return self._convert(packed, "backwardu")

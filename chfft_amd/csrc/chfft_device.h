@@ -352,9 +352,16 @@ __device__ void store_stage(const KParams& P, const TMap& m, void* dst, const cx
         if (P.conj_out) v = cj(v);
         out[k * es] = scl(v, s);
       } else if constexpr (ST == ST_TW) {
-        // table of W^(c*k) laid out so that consecutive lanes read
-        // consecutive entries: index k * twk + c * twc
-        v = v * reinterpret_cast<const cx<T>*>(P.tab_a)[k * P.twk + c * P.twc];
+        if (P.twsh) {
+          // two-level table: W^m = W^(m_hi << sh) * W^(m_lo), both rounded once
+          const int mm = c * k;
+          v = v * (reinterpret_cast<const cx<T>*>(P.tab_a)[mm >> P.twsh] *
+                   reinterpret_cast<const cx<T>*>(P.tab_b)[mm & ((1 << P.twsh) - 1)]);
+        } else {
+          // table of W^(c*k) laid out so that consecutive lanes read
+          // consecutive entries: index k * twk + c * twc
+          v = v * reinterpret_cast<const cx<T>*>(P.tab_a)[k * P.twk + c * P.twc];
+        }
         if (P.conj_out) v = cj(v);
         out[k * es] = scl(v, s);
       } else {

@@ -17,6 +17,9 @@ constexpr int kMaxThreadsF64 = 512;
 // ... and of the persistent four-step kernel (more live state per thread)
 constexpr int kMaxThreadsFsF32 = 512;
 constexpr int kMaxThreadsFsF64 = 256;
+// ... and of the multi-length persistent kernel
+constexpr int kMultiThreadsF32 = 512;
+constexpr int kMultiThreadsF64 = 256;
 // LDS per workgroup on gfx950.
 constexpr size_t kLdsBytes = 160 * 1024;
 
@@ -92,11 +95,33 @@ struct KParams {
   long long N;      // logical length the operator works on
   long long twN;    // four-step twiddle: size of the W^(c*k) table
   int twk, twc;     // four-step twiddle index = k * twk + c * twc
+  int twsh;         // > 0: two-level twiddle W^m = tab_a[m >> twsh] * tab_b[m & (2^twsh - 1)], m = c * k
   long long kc, ke; // spectrum index = c * kc + idx * ke
   const void* chirp;
   const void* kspec;
   const void* tab_a;
   const void* tab_b;
+};
+
+// One item of a multi-length batch (one transform). kind 0: fused (descs[desc]);
+// 1: four-step (descs[desc], descs[desc+1]); 2: four-step Bluestein (three
+// descriptors); 3: length 1 (copy).
+struct MultiItem {
+  long long off;  // element offset of the transform in the in/out buffers
+  int n;
+  int kind;
+  int desc;
+  int pad;
+};
+
+// Device table generation job (chfft_tables.hip).
+struct RootJob {
+  long long off;
+  long long count;
+  long long den;
+  long long p;
+  int kind;
+  int sh;
 };
 
 }  // namespace chfft

@@ -198,6 +198,60 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? kMaxThreadsFsF64 : kMaxThread
   }
 }
 
+// Uneven multi-length batch in one persistent launch: workgroup w takes items
+// w, w + grid, w + 2*grid, ... (the planner sorts them by decreasing cost) and
+// runs each whole: in LDS when it fits, else as a four-step [Bluestein]
+// through the workgroup's scratch slot. Every loop bound is the item count,
+// so every wave exits.
+template <typename T>
+__global__ void __launch_bounds__(sizeof(T) == 8 ? kMultiThreadsF64 : kMultiThreadsF32)
+    multi_kernel(const MultiItem* __restrict__ items, int nitems, const KParams* __restrict__ descs, const void* in,
+                 void* out, int scale_kind, void* scratch, long long slot_elems) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* slot = reinterpret_cast<cx<T>*>(scratch) + (long long)blockIdx.x * slot_elems;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const MultiItem m = items[it];
+    const cx<T>* src = reinterpret_cast<const cx<T>*>(in) + m.off;
+    cx<T>* dst = reinterpret_cast<cx<T>*>(out) + m.off;
+    const double n = (double)m.n;
+    const double scale = scale_kind == 0 ? 1.0 : scale_kind == 1 ? 1.0 / sqrt(n) : 1.0 / n;
+    if (m.kind == 0) {
+      block_transform<T, LD_PLAIN, ST_PLAIN>(descs[m.desc], src, dst, lds, 0, 1, scale);
+      __syncthreads();
+    } else if (m.kind == 1) {
+      run_phase<T, LD_PLAIN, ST_TW>(descs[m.desc], src, slot, lds, 1.0);
+      run_phase<T, LD_PLAIN, ST_PLAIN>(descs[m.desc + 1], slot, dst, lds, scale);
+    } else if (m.kind == 2) {
+      run_phase<T, LD_FS_CHIRP, ST_TW>(descs[m.desc], src, slot, lds, 1.0);
+      run_phase<T, LD_PLAIN, ST_TW>(descs[m.desc + 1], slot, slot, lds, 1.0);
+      run_phase<T, LD_PLAIN, ST_FS_DECHIRP>(descs[m.desc + 2], slot, dst, lds, scale);
+    } else {
+      // length 1: returned unchanged (cfft1d.rs:213-214)
+      if (threadIdx.x == 0) dst[0] = src[0];
+      __syncthreads();
+    }
+  }
+}
+
+hipError_t launch_multi(bool f64, const MultiItem* items, int nitems, const KParams* descs, const void* in, void* out,
+                        int scale_kind, void* scratch, long long slot_elems, dim3 grid, size_t lds, hipStream_t st) {
+  if (f64)
+    hipLaunchKernelGGL(multi_kernel<double>, grid, dim3(kMultiThreadsF64), lds, st, items, nitems, descs, in, out,
+                       scale_kind, scratch, slot_elems);
+  else
+    hipLaunchKernelGGL(multi_kernel<float>, grid, dim3(kMultiThreadsF32), lds, st, items, nitems, descs, in, out,
+                       scale_kind, scratch, slot_elems);
+  return hipGetLastError();
+}
+
+int multi_occupancy(bool f64, size_t lds) {
+  int n = 0;
+  hipError_t e = f64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, multi_kernel<double>, kMultiThreadsF64, lds)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, multi_kernel<float>, kMultiThreadsF32, lds);
+  return (e == hipSuccess && n > 0) ? n : 1;
+}
+
 // Plain copy used for length-1 transforms (cfft1d.rs:213-214 returns the input
 // unchanged, whatever the scaling).
 template <typename T>
@@ -261,6 +315,12 @@ hipError_t set_fused_lds_limits() {
                        reinterpret_cast<const void*>(&fourstep_kernel<float, 3>),
                        reinterpret_cast<const void*>(&fourstep_kernel<double, 2>),
                        reinterpret_cast<const void*>(&fourstep_kernel<double, 3>)};
+  const void* ms[2] = {reinterpret_cast<const void*>(&multi_kernel<float>),
+                       reinterpret_cast<const void*>(&multi_kernel<double>)};
+  for (const void* f : ms) {
+    hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes);
+    if (r != hipSuccess) e = r;
+  }
   for (const void* f : fs) {
     hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes);
     if (r != hipSuccess) e = r;

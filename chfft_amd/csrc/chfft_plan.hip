@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -39,6 +40,13 @@ hipError_t launch_fourstep(bool f64, int nph, const KParams* Q, const void* in, 
                            long long src_dist, long long dst_dist, void* scratch, long long slot_elems, dim3 grid,
                            dim3 block, size_t lds, hipStream_t stream);
 int fourstep_occupancy(bool f64, int nph, int block, size_t lds);
+hipError_t launch_multi(bool f64, const MultiItem* items, int nitems, const KParams* descs, const void* in, void* out,
+                        int scale_kind, void* scratch, long long slot_elems, dim3 grid, size_t lds, hipStream_t st);
+int multi_occupancy(bool f64, size_t lds);
+hipError_t gen_roots(bool f64, const RootJob* jobs_dev, int njobs, void* arena, hipStream_t st);
+hipError_t gen_rot(const long long* ns_dev, int count, long long M, void* out, hipStream_t st);
+hipError_t put_spectra(bool f64, const void* src, int count, long long M, const long long* offs_dev, void* arena,
+                       hipStream_t st);
 struct SfftEntry {
   int f64, L, tpt, tpb, col, ld, st, mode, epi;
   int nrad;
@@ -305,7 +313,7 @@ Geometry choose_geometry(int Lfft, long long G, bool colmajor, bool f64, int max
 }
 
 // ------------------------------------------------------------------ plan objects
-enum Kind { K_C2C = 0, K_R2C = 1, K_C2C2D = 2, K_DCT = 3, K_MDCT = 4 };
+enum Kind { K_C2C = 0, K_R2C = 1, K_C2C2D = 2, K_DCT = 3, K_MDCT = 4, K_MULTI = 5 };
 enum Slot { S_IN = 0, S_OUT = 1, S_SCRATCH = 2 };
 
 struct Launch {
@@ -836,6 +844,15 @@ struct chfft_plan_s {
   DevBuf stage_in, stage_out;
   std::vector<std::unique_ptr<DevBuf>> qbufs;  // four-step phase descriptors
   std::vector<Launch> fwd, bwd;
+  // multi-length batch (K_MULTI)
+  std::vector<size_t> lens;
+  size_t total = 0;  // complex elements over all transforms
+  int nitems = 0;
+  DevBuf arena, items_dev, descs_fwd, descs_bwd;
+  long long slot_elems = 0;
+  size_t multi_lds = 0;
+  int multi_grid = 0, multi_occ = 1;
+  std::string multi_desc;
 };
 
 namespace chfft {
@@ -1174,6 +1191,16 @@ namespace {
 int method_scale(const chfft_plan_s* p, int method, bool* inverse, double* scale) {
   const double N = p->kind == K_C2C2D ? (double)p->m * (double)p->n : (double)p->n;
   switch (p->kind) {
+    case K_MULTI:  // scale kind (per transform length): 0 -> 1, 1 -> 1/sqrt(n), 2 -> 1/n
+      switch (method) {
+        case CHFFT_FORWARD: case CHFFT_FORWARD0: *inverse = false; *scale = 0; return CHFFT_OK;
+        case CHFFT_FORWARDU: *inverse = false; *scale = 1; return CHFFT_OK;
+        case CHFFT_FORWARDN: *inverse = false; *scale = 2; return CHFFT_OK;
+        case CHFFT_BACKWARD: case CHFFT_BACKWARDN: *inverse = true; *scale = 2; return CHFFT_OK;
+        case CHFFT_BACKWARD0: *inverse = true; *scale = 0; return CHFFT_OK;
+        case CHFFT_BACKWARDU: *inverse = true; *scale = 1; return CHFFT_OK;
+      }
+      return CHFFT_ERR_INVALID_ARGUMENT;
     case K_C2C:
     case K_R2C:
     case K_C2C2D:
@@ -1219,6 +1246,7 @@ void io_bytes(const chfft_plan_s* p, bool inverse, size_t* ib, size_t* ob) {
   const size_t e = p->f64 ? 8 : 4;  // bytes per real
   const size_t B = p->batch, n = p->n;
   switch (p->kind) {
+    case K_MULTI: *ib = *ob = 2 * e * p->total; return;
     case K_C2C: *ib = *ob = 2 * e * n * B; return;
     case K_C2C2D: *ib = *ob = 2 * e * n * p->m * B; return;
     case K_R2C:
@@ -1255,6 +1283,16 @@ int chfft_exec_range(chfft_plan plan, const void* in, void* out, int method, int
   int rc = method_scale(plan, method, &inverse, &scale);
   if (rc) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (plan->kind == K_MULTI) {
+    if (first != 0 || count != 1) return count == 0 ? CHFFT_OK : CHFFT_ERR_INVALID_ARGUMENT;
+    if (plan->nitems == 0) return CHFFT_OK;
+    const KParams* descs = reinterpret_cast<const KParams*>(inverse ? plan->descs_bwd.p : plan->descs_fwd.p);
+    if (launch_multi(plan->f64, reinterpret_cast<const MultiItem*>(plan->items_dev.p), plan->nitems, descs, in, out,
+                     (int)scale, plan->scratch.p, plan->slot_elems, dim3((unsigned)plan->multi_grid),
+                     plan->multi_lds, st) != hipSuccess)
+      return CHFFT_ERR_HIP;
+    return CHFFT_OK;
+  }
   const std::vector<Launch>& L = inverse ? plan->bwd : plan->fwd;
   if (first < 0 || count < 0 || (size_t)first + (size_t)count > L.size()) return CHFFT_ERR_INVALID_ARGUMENT;
   for (int li = first; li < first + count; li++) {
@@ -1313,6 +1351,7 @@ int chfft_plan_launch_count(chfft_plan plan, int method) {
   double scale;
   int rc = method_scale(plan, method, &inverse, &scale);
   if (rc) return -rc;
+  if (plan->kind == K_MULTI) return 1;
   return (int)(inverse ? plan->bwd.size() : plan->fwd.size());
 }
 
@@ -1343,6 +1382,7 @@ int chfft_exec_host(chfft_plan plan, const void* in, void* out, int method) {
 int chfft_plan_describe(chfft_plan plan, char* buf, size_t buf_len) {
   if (!plan) return -1;
   std::string s;
+  if (plan->kind == K_MULTI) s = plan->multi_desc;
   for (int dir = 0; dir < 2; dir++) {
     const std::vector<Launch>& L = dir ? plan->bwd : plan->fwd;
     if (L.empty()) continue;
@@ -1359,7 +1399,478 @@ int chfft_plan_describe(chfft_plan plan, char* buf, size_t buf_len) {
     memcpy(buf, s.data(), k);
     buf[k] = 0;
   }
-  return (int)plan->fwd.size();
+  return plan->kind == K_MULTI ? 1 : (int)plan->fwd.size();
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- multi-length batch
+namespace chfft {
+namespace {
+
+struct SeqTab {  // a pass sequence whose table lives in the arena
+  SeqDesc seq{};
+  long long off = 0;  // arena offset of its table
+  size_t count = 0;
+};
+
+struct LenInfo {
+  int kind = 0;  // 0 fused, 1 four-step, 2 four-step Bluestein, 3 copy
+  size_t n = 0, Lfft = 0, M = 0, n1 = 0, n2 = 0;
+  bool blue = false;
+  int seq = -1, s1 = -1, s2 = -1;  // indices into the SeqTab list
+  long long chirp = -1, kspec = -1, tw_hi = -1, tw_lo = -1;
+  int twsh = 0;
+  int desc = 0;  // first descriptor of this length (per direction)
+  int ndesc = 0;
+};
+
+int pow2_ceil_log(size_t n) {
+  int b = 0;
+  while (((size_t)1 << b) < n) b++;
+  return b;
+}
+
+// tpb for a phase of length L under a fixed workgroup of `block` threads:
+// column phases want 16 transforms (128-byte rows), all need tpt >= L/16.
+int phase_tpb(size_t L, int block, bool colmajor, bool f64) {
+  int tpb = colmajor ? 16 : 16;
+  const size_t esz = f64 ? 16 : 8;
+  while (tpb > 1) {
+    const int tpt = block / tpb;
+    size_t Lp = L + (L + 15) / 16;
+    if (tpb > 1 && (Lp & 1) == 0) Lp++;
+    if ((size_t)tpt * kEmax >= L && (size_t)tpb * Lp * esz <= kLdsBytes) break;
+    tpb >>= 1;
+  }
+  return tpb;
+}
+
+int lp_for(size_t L, int tpb) {
+  size_t Lp = L + (L + 15) / 16;
+  if (tpb > 1 && (Lp & 1) == 0) Lp++;
+  return (int)Lp;
+}
+
+}  // namespace
+}  // namespace chfft
+
+extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t count, int precision) {
+  if (!plan) return CHFFT_ERR_INVALID_ARGUMENT;
+  *plan = nullptr;
+  if (check_prec(precision)) return CHFFT_ERR_INVALID_ARGUMENT;
+  if (count > 0 && !lens) return CHFFT_ERR_INVALID_ARGUMENT;
+  size_t total = 0;
+  for (size_t i = 0; i < count; i++) {
+    if (lens[i] == 0) return CHFFT_ERR_INVALID_LENGTH;
+    total += lens[i];
+  }
+  if (total > ((size_t)1 << 31)) return CHFFT_ERR_INVALID_ARGUMENT;
+  int rc = ensure_runtime();
+  if (rc) return rc;
+  const bool f64 = precision == CHFFT_F64;
+  const size_t esz = esz_of(f64);
+  const int block = f64 ? kMultiThreadsF64 : kMultiThreadsF32;
+  const size_t capM = (size_t)kEmax * block;  // one workgroup, tpb = 1
+  std::unique_ptr<chfft_plan_s> p(new chfft_plan_s());
+  p->kind = K_MULTI;
+  p->f64 = f64;
+  p->lens.assign(lens, lens + count);
+  p->total = total;
+  p->batch = 1;
+
+  // ---- distinct lengths and their schedules
+  std::vector<SeqTab> seqs;
+  std::map<size_t, int> seq_of_len;
+  long long arena = 0;
+  auto seq_for = [&](size_t L) {
+    auto it = seq_of_len.find(L);
+    if (it != seq_of_len.end()) return it->second;
+    SeqTab t;
+    std::vector<int> rad = radices_for(L);
+    t.seq.L = (int)L;
+    t.seq.npass = (int)rad.size();
+    int pp = 1;
+    size_t cnt = 0;
+    for (size_t q = 0; q < rad.size(); q++) {
+      PassDesc& pd = t.seq.pass[q];
+      pd.radix = rad[q];
+      pd.p = pp;
+      pd.tws = (int)(L / ((size_t)pp * rad[q]));
+      pd.pmagic = (unsigned)((((unsigned long long)1 << 32) + (unsigned long long)pp - 1) / (unsigned long long)pp);
+      pd.twoff = (int)cnt;
+      if (pp > 1) cnt += (size_t)(rad[q] - 1) * pp;
+      pp *= rad[q];
+    }
+    t.count = cnt ? cnt : 1;
+    t.off = arena;
+    arena += (long long)t.count;
+    seqs.push_back(t);
+    seq_of_len[L] = (int)seqs.size() - 1;
+    return (int)seqs.size() - 1;
+  };
+  std::map<size_t, LenInfo> info;
+  std::vector<RootJob> jobs;
+  for (size_t i = 0; i < count; i++) {
+    const size_t n = lens[i];
+    if (info.count(n)) continue;
+    LenInfo L;
+    L.n = n;
+    if (n == 1) {
+      L.kind = 3;
+    } else {
+      const bool sm = smooth7(n);
+      const size_t M = sm ? n : 2 * next_pow2(n);
+      L.blue = !sm;
+      L.M = M;
+      if (M <= capM) {
+        L.kind = 0;
+        L.Lfft = M;
+        L.seq = seq_for(M);
+      } else {
+        L.kind = sm ? 1 : 2;
+        size_t best = 0;
+        bool best_aligned = false;
+        for (size_t d = 1; d * d <= M; d++) {
+          if (M % d || !smooth7(d) || !smooth7(M / d)) continue;
+          const bool al = (M / d) % 16 == 0;
+          if (al || !best_aligned) {
+            best = d;
+            best_aligned = al;
+          }
+        }
+        if (!best || M / best > capM) return CHFFT_ERR_UNSUPPORTED;
+        L.n1 = best;
+        L.n2 = M / best;
+        L.s1 = seq_for(L.n1);
+        L.s2 = seq_for(L.n2);
+        L.twsh = (pow2_ceil_log(M) + 1) / 2;
+        const size_t nhi = (M + ((size_t)1 << L.twsh) - 1) >> L.twsh, nlo = (size_t)1 << L.twsh;
+        L.tw_hi = arena;
+        jobs.push_back({arena, (long long)nhi, (long long)M, 1, 2, L.twsh});
+        arena += (long long)nhi;
+        L.tw_lo = arena;
+        jobs.push_back({arena, (long long)nlo, (long long)M, 1, 3, 0});
+        arena += (long long)nlo;
+      }
+      if (L.blue) {
+        L.chirp = arena;
+        jobs.push_back({arena, (long long)n, (long long)(2 * n), 1, 1, 0});
+        arena += (long long)n;
+        L.kspec = arena;
+        arena += (long long)M;
+      }
+    }
+    info[n] = L;
+  }
+  for (const SeqTab& t : seqs) {
+    int pp = 1;
+    for (int q = 0; q < t.seq.npass; q++) {
+      const PassDesc& pd = t.seq.pass[q];
+      if (pp > 1)
+        jobs.push_back({t.off + pd.twoff, (long long)(pd.radix - 1) * pp, (long long)pp * pd.radix, (long long)pp, 0,
+                        0});
+      pp *= pd.radix;
+    }
+  }
+
+  // ---- tables on the device (FP64 math, rounded once)
+  if ((rc = dev_alloc(p->arena, (size_t)std::max<long long>(arena, 1) * esz))) return rc;
+  {
+    DevBuf jd;
+    if ((rc = dev_alloc(jd, std::max<size_t>(jobs.size(), 1) * sizeof(RootJob)))) return rc;
+    if (!jobs.empty() && hipMemcpy(jd.p, jobs.data(), jobs.size() * sizeof(RootJob), hipMemcpyHostToDevice) != hipSuccess)
+      return CHFFT_ERR_HIP;
+    if (gen_roots(f64, reinterpret_cast<const RootJob*>(jd.p), (int)jobs.size(), p->arena.p, nullptr) != hipSuccess)
+      return CHFFT_ERR_HIP;
+    if (hipDeviceSynchronize() != hipSuccess) return CHFFT_ERR_HIP;
+  }
+  // Bluestein spectra: FP64 FFT of the folded chirp, grouped by M, in chunks
+  {
+    std::map<size_t, std::vector<size_t>> by_m;
+    for (auto& kv : info)
+      if (kv.second.blue && kv.second.kind != 3) by_m[kv.second.M].push_back(kv.first);
+    const size_t chunk_bytes = (size_t)1 << 30;
+    for (auto& kv : by_m) {
+      const size_t M = kv.first;
+      const std::vector<size_t>& ns = kv.second;
+      const size_t per = std::max<size_t>(1, chunk_bytes / (M * 16));
+      for (size_t b0 = 0; b0 < ns.size(); b0 += per) {
+        const size_t cnt = std::min(per, ns.size() - b0);
+        std::vector<long long> nv(cnt), offs(cnt);
+        for (size_t k = 0; k < cnt; k++) {
+          nv[k] = (long long)ns[b0 + k];
+          offs[k] = info[ns[b0 + k]].kspec;
+        }
+        DevBuf nd, od, buf;
+        if ((rc = dev_alloc(nd, cnt * 8)) || (rc = dev_alloc(od, cnt * 8)) || (rc = dev_alloc(buf, cnt * M * 16)))
+          return rc;
+        if (hipMemcpy(nd.p, nv.data(), cnt * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(od.p, offs.data(), cnt * 8, hipMemcpyHostToDevice) != hipSuccess)
+          return CHFFT_ERR_HIP;
+        if (gen_rot(reinterpret_cast<const long long*>(nd.p), (int)cnt, (long long)M, buf.p, nullptr) != hipSuccess)
+          return CHFFT_ERR_HIP;
+        chfft_plan sub = nullptr;
+        if ((rc = chfft_plan_c2c_1d(&sub, M, cnt, CHFFT_F64))) return rc;
+        rc = chfft_exec(sub, buf.p, buf.p, CHFFT_FORWARD, nullptr);
+        if (!rc && put_spectra(f64, buf.p, (int)cnt, (long long)M, reinterpret_cast<const long long*>(od.p),
+                               p->arena.p, nullptr) != hipSuccess)
+          rc = CHFFT_ERR_HIP;
+        if (!rc && hipDeviceSynchronize() != hipSuccess) rc = CHFFT_ERR_HIP;
+        chfft_plan_destroy(sub);
+        if (rc) return rc;
+      }
+    }
+  }
+
+  // ---- descriptors (per distinct length, both directions) and items
+  const char* base = reinterpret_cast<const char*>(p->arena.p);
+  auto at = [&](long long off) { return (const void*)(base + (size_t)off * esz); };
+  std::vector<KParams> dfw, dbw;
+  size_t lds = 0;
+  long long slot = 0;
+  for (auto& kv : info) {
+    LenInfo& L = kv.second;
+    if (L.kind == 3) continue;
+    L.desc = (int)dfw.size();
+    for (int dir = 0; dir < 2; dir++) {
+      std::vector<KParams>& D = dir ? dbw : dfw;
+      if (L.kind == 0) {
+        KParams P{};
+        P.seq = seqs[L.seq].seq;
+        P.seq.tw = at(seqs[L.seq].off);
+        P.Lfft = (int)L.Lfft;
+        P.L = (int)L.n;
+        P.G = 1;
+        P.tpb = 1;
+        P.Lp = lp_for(L.Lfft, 1);
+        P.in = io(0, 0, 1, 1);
+        P.out = P.in;
+        P.conj_in = P.conj_out = dir;
+        P.has_mid = P.has_chirp = L.blue ? 1 : 0;
+        if (L.blue) {
+          P.chirp = at(L.chirp);
+          P.kspec = at(L.kspec);
+          P.kc = 0;
+          P.ke = 1;
+        }
+        lds = std::max(lds, (size_t)P.Lp * esz);
+        D.push_back(P);
+      } else {
+        const int ta = phase_tpb(L.n1, block, true, f64), tb = phase_tpb(L.n2, block, L.kind == 1, f64);
+        KParams A{}, B{}, C{};
+        A.seq = seqs[L.s1].seq;
+        A.seq.tw = at(seqs[L.s1].off);
+        A.Lfft = A.L = (int)L.n1;
+        A.G = (int)L.n2;
+        A.tpb = ta;
+        A.Lp = lp_for(L.n1, ta);
+        A.in = io(0, 1, (long long)L.n2, (int)L.n2);
+        A.out = A.in;
+        A.colmajor_ld = A.colmajor_st = 1;
+        A.conj_in = dir;
+        A.twsh = L.twsh;
+        A.tab_a = at(L.tw_hi);
+        A.tab_b = at(L.tw_lo);
+        B.seq = seqs[L.s2].seq;
+        B.seq.tw = at(seqs[L.s2].off);
+        B.Lfft = B.L = (int)L.n2;
+        B.G = (int)L.n1;
+        B.tpb = tb;
+        B.Lp = lp_for(L.n2, tb);
+        B.in = io(0, (long long)L.n2, 1, (int)L.n1);
+        if (L.kind == 1) {
+          B.out = io(0, 1, (long long)L.n1, (int)L.n1);
+          B.colmajor_st = 1;
+          B.conj_out = dir;
+        } else {
+          A.N = (long long)L.n;
+          A.chirp = at(L.chirp);
+          B.out = B.in;
+          B.has_mid = 1;
+          B.kspec = at(L.kspec);
+          B.kc = 1;
+          B.ke = (long long)L.n1;
+          B.twsh = L.twsh;
+          B.tab_a = at(L.tw_hi);
+          B.tab_b = at(L.tw_lo);
+          C = A;
+          C.conj_in = 0;
+          C.twsh = 0;
+          C.tab_a = C.tab_b = nullptr;
+          C.out = io(0, 1, (long long)L.n2, (int)L.n2);
+          C.N = (long long)L.n;
+          C.chirp = at(L.chirp);
+          C.conj_out = dir;
+        }
+        lds = std::max(lds, (size_t)ta * A.Lp * esz);
+        lds = std::max(lds, (size_t)tb * B.Lp * esz);
+        D.push_back(A);
+        D.push_back(B);
+        if (L.kind == 2) D.push_back(C);
+        slot = std::max(slot, (long long)L.M);
+      }
+    }
+    L.ndesc = (int)dfw.size() - L.desc;
+  }
+  // items: offsets in the packed buffers, largest work first
+  std::vector<MultiItem> items(count);
+  std::vector<size_t> off(count);
+  size_t acc = 0;
+  for (size_t i = 0; i < count; i++) {
+    off[i] = acc;
+    acc += lens[i];
+  }
+  std::vector<size_t> order(count);
+  for (size_t i = 0; i < count; i++) order[i] = i;
+  auto cost = [&](size_t i) {
+    const LenInfo& L = info[lens[i]];
+    const double m = (double)std::max<size_t>(L.M, 1);
+    return m * std::log2(m + 1) * (L.kind == 2 ? 3.0 : L.kind == 1 ? 2.0 : L.blue ? 2.0 : 1.0);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
+  for (size_t k = 0; k < count; k++) {
+    const size_t i = order[k];
+    const LenInfo& L = info[lens[i]];
+    items[k].off = (long long)off[i];
+    items[k].n = (int)lens[i];
+    items[k].kind = L.kind;
+    items[k].desc = L.desc;
+  }
+  if ((rc = dev_alloc(p->items_dev, std::max<size_t>(count, 1) * sizeof(MultiItem)))) return rc;
+  if ((rc = dev_alloc(p->descs_fwd, std::max<size_t>(dfw.size(), 1) * sizeof(KParams)))) return rc;
+  if ((rc = dev_alloc(p->descs_bwd, std::max<size_t>(dbw.size(), 1) * sizeof(KParams)))) return rc;
+  if ((count && hipMemcpy(p->items_dev.p, items.data(), count * sizeof(MultiItem), hipMemcpyHostToDevice)) ||
+      (!dfw.empty() && hipMemcpy(p->descs_fwd.p, dfw.data(), dfw.size() * sizeof(KParams), hipMemcpyHostToDevice)) ||
+      (!dbw.empty() && hipMemcpy(p->descs_bwd.p, dbw.data(), dbw.size() * sizeof(KParams), hipMemcpyHostToDevice)))
+    return CHFFT_ERR_HIP;
+  p->nitems = (int)count;
+  p->multi_lds = std::max<size_t>(lds, 16);
+  p->multi_occ = multi_occupancy(f64, p->multi_lds);
+  p->multi_grid = std::max(1, std::min((int)count, p->multi_occ * C2CEngine::num_cus()));
+  p->slot_elems = slot;
+  if ((rc = dev_alloc(p->scratch, (size_t)p->multi_grid * (size_t)slot * esz))) return rc;
+  size_t nf = 0, nf4 = 0, nb4 = 0;
+  for (auto& kv : info) {
+    nf += kv.second.kind == 0;
+    nf4 += kv.second.kind == 1;
+    nb4 += kv.second.kind == 2;
+  }
+  p->multi_desc = "multi-length batch: " + std::to_string(count) + " transforms, " + std::to_string(total) +
+                  " points, " + std::to_string(info.size()) + " distinct lengths (" + std::to_string(nf) +
+                  " fused in LDS, " + std::to_string(nf4) + " four-step, " + std::to_string(nb4) +
+                  " four-step Bluestein) | one persistent launch: grid=" + std::to_string(p->multi_grid) +
+                  " block=" + std::to_string(block) + " lds=" + std::to_string(p->multi_lds) +
+                  " scratch=" + std::to_string((size_t)p->multi_grid * (size_t)slot * esz) + "\n";
+  *plan = p.release();
+  return CHFFT_OK;
+}
+
+// Sets the number of workgroups of a multi-length plan's persistent launch
+// (the autotuner's knob); 0 restores the occupancy-based default.
+extern "C" int chfft_plan_set_grid(chfft_plan plan, int grid) {
+  if (!plan || plan->kind != K_MULTI || grid < 0) return CHFFT_ERR_INVALID_ARGUMENT;
+  const int def = std::max(1, std::min(plan->nitems, plan->multi_occ * C2CEngine::num_cus()));
+  const int g = grid ? grid : def;
+  if ((size_t)g * (size_t)plan->slot_elems * esz_of(plan->f64) > plan->scratch.bytes) {
+    int rc = dev_alloc(plan->scratch, (size_t)g * (size_t)plan->slot_elems * esz_of(plan->f64));
+    if (rc) return rc;
+  }
+  plan->multi_grid = g;
+  return CHFFT_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+namespace chfft {
+namespace {
+
+// Rebuild the launches of a c2c plan with a four-step schedule override.
+int rebuild_c2c(chfft_plan_s* p, bool split, bool persistent) {
+  p->rows.force_split = split;
+  p->rows.force_persistent = persistent;
+  p->fwd.clear();
+  p->bwd.clear();
+  p->qbufs.clear();
+  p->rows.emit(p->fwd, p->batch, false, S_IN, S_OUT, (long long)p->n, (long long)p->n);
+  p->rows.emit(p->bwd, p->batch, true, S_IN, S_OUT, (long long)p->n, (long long)p->n);
+  size_t need = p->rows.scratch_elems(p->batch);
+  need = std::max(need, p->rows.persistent_scratch_elems(p->fwd));
+  need = std::max(need, p->rows.persistent_scratch_elems(p->bwd));
+  int rc = dev_alloc(p->scratch, need * esz_of(p->f64));
+  if (rc) return rc;
+  for (std::vector<Launch>* L : {&p->fwd, &p->bwd})
+    for (Launch& l : *L)
+      if (l.kind == 2 || l.kind == 4) {
+        p->qbufs.emplace_back(new DevBuf());
+        DevBuf& q = *p->qbufs.back();
+        if ((rc = dev_alloc(q, sizeof(KParams) * l.nph))) return rc;
+        if (hipMemcpy(q.p, l.Q, sizeof(KParams) * l.nph, hipMemcpyHostToDevice) != hipSuccess) return CHFFT_ERR_HIP;
+        l.qdev = q.p;
+      }
+  return CHFFT_OK;
+}
+
+double time_exec(chfft_plan_s* p, void* in, void* out, int reps, hipStream_t st) {
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1;
+  double ms = -1;
+  if (chfft_exec(p, in, out, CHFFT_FORWARD, st) == CHFFT_OK && hipEventRecord(a, st) == hipSuccess) {
+    bool ok = true;
+    for (int r = 0; r < reps && ok; r++) ok = chfft_exec(p, in, out, CHFFT_FORWARD, st) == CHFFT_OK;
+    float f = 0;
+    if (ok && hipEventRecord(b, st) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+        hipEventElapsedTime(&f, a, b) == hipSuccess)
+      ms = f / reps;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms;
+}
+
+}  // namespace
+}  // namespace chfft
+
+extern "C" int chfft_plan_autotune(chfft_plan plan, void* in, void* out, int reps, void* stream) {
+  if (!plan || !in || !out || reps <= 0) return -CHFFT_ERR_INVALID_ARGUMENT;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  std::string report = "autotune:";
+  int best = -1;
+  double best_ms = 1e30;
+  if (plan->kind == K_MULTI) {
+    const int def = std::max(1, std::min(plan->nitems, plan->multi_occ * C2CEngine::num_cus()));
+    const int cand[5] = {def / 4, def / 2, def, def * 2, def * 4};
+    for (int i = 0; i < 5; i++) {
+      const int g = std::max(1, std::min(cand[i], std::max(1, plan->nitems)));
+      if (chfft_plan_set_grid(plan, g) != CHFFT_OK) continue;
+      const double ms = time_exec(plan, in, out, reps, st);
+      report += " grid=" + std::to_string(g) + ":" + std::to_string(ms) + "ms";
+      if (ms > 0 && ms < best_ms) {
+        best_ms = ms;
+        best = i;
+      }
+    }
+    if (best < 0) return -CHFFT_ERR_HIP;
+    chfft_plan_set_grid(plan, std::max(1, std::min(cand[best], std::max(1, plan->nitems))));
+    plan->multi_desc += report + " -> grid " + std::to_string(plan->multi_grid) + "\n";
+    return best;
+  }
+  if (plan->kind == K_C2C && (plan->rows.mode == 2 || plan->rows.mode == 3)) {
+    const bool opts[2][2] = {{true, false}, {false, true}};
+    const char* names[2] = {"split", "persistent"};
+    for (int i = 0; i < 2; i++) {
+      if (rebuild_c2c(plan, opts[i][0], opts[i][1]) != CHFFT_OK) continue;
+      const double ms = time_exec(plan, in, out, reps, st);
+      report += std::string(" ") + names[i] + ":" + std::to_string(ms) + "ms";
+      if (ms > 0 && ms < best_ms) {
+        best_ms = ms;
+        best = i;
+      }
+    }
+    if (best < 0) return -CHFFT_ERR_HIP;
+    int rc = rebuild_c2c(plan, opts[best][0], opts[best][1]);
+    if (rc) return -rc;
+    if (!plan->fwd.empty()) plan->fwd.back().what += " [" + report + "]";
+    return best;
+  }
+  return 0;  // a single schedule: nothing to choose
+}

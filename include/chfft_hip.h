@@ -93,6 +93,28 @@ int chfft_plan_dct_1d(chfft_plan* plan, int dct_type, size_t len, int is_ortho, 
 int chfft_plan_mdct_1d(chfft_plan* plan, size_t len, const double* window, size_t batch, size_t frame_stride,
                        int precision);
 
+/* Uneven batch (no counterpart type in the reference; each transform is the
+   reference's CFft1D::forward/backward of its own length, cfft1d.rs:208-230):
+   `count` complex transforms of lengths lens[0..count), packed back to back
+   in one buffer (transform i starts at element sum(lens[0..i))). One exec is
+   ONE persistent kernel launch: each workgroup takes whole transforms, in LDS
+   when they fit, else as a four-step (Bluestein for non-7-smooth lengths)
+   through its own scratch slot. Tables are generated on the GPU in FP64 and
+   rounded once; Bluestein spectra by an FP64 FFT. Methods scale each
+   transform by its own length, as CFft1D does. */
+int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t count, int precision);
+
+/* Workgroup count of a multi-length plan's persistent launch (0 = default). */
+int chfft_plan_set_grid(chfft_plan plan, int grid);
+
+/* Autotuner (no counterpart in the reference): times the schedules this plan
+   can run — for a multi-length plan the persistent launch's workgroup count,
+   for a four-step length the split launches against the persistent
+   one-transform-per-workgroup launch — on the given device buffers (their
+   contents are overwritten), keeps the fastest and returns its index
+   (>= 0) or minus a status code. `reps` timed executions per candidate. */
+int chfft_plan_autotune(chfft_plan plan, void* in, void* out, int reps, void* stream);
+
 int chfft_plan_destroy(chfft_plan plan);
 
 /* Run `method` on device buffers, asynchronously on `stream`. */
