@@ -429,6 +429,18 @@ void fill_inner(KParams& P, const Inner& in) {
   P.ke = 1;
 }
 
+// Largest byte offset of one tile's accesses (tpb consecutive transforms of
+// len elements under d) from the tile's first transform; -1 when an offset
+// could be negative (transform bases not increasing with g).
+long long tile_span_bytes(const IoDesc& d, long long G, int tpb, long long len, size_t esz) {
+  if (d.bdist < 0 || d.cstride < 0 || d.estride < 0 || d.ncols < 1) return -1;
+  if (d.ncols > 1 && d.bdist < (long long)(d.ncols - 1) * d.cstride) return -1;
+  auto base = [&](long long g) { return (g / d.ncols) * d.bdist + (g % d.ncols) * d.cstride; };
+  long long worst = 0;
+  for (long long g0 = 0; g0 < G; g0 += tpb) worst = std::max(worst, base(std::min(G, g0 + tpb) - 1) - base(g0));
+  return (worst + (len - 1) * d.estride + 1) * (long long)esz;
+}
+
 // Use a length-specialised kernel for a fused launch when one is registered
 // for this length, operator pair and radix schedule.
 void try_specialise(Launch& l, bool f64) {
@@ -441,6 +453,24 @@ void try_specialise(Launch& l, bool f64) {
     if (e->rad[q] != l.P.seq.pass[q].radix) return;
   // a direct (non-epilogue) store uses the load's thread layout
   if (!e->epi && l.P.colmajor_st != l.P.colmajor_ld) return;
+  // plain loads and plain / twiddled stores address each tile by 32-bit
+  // byte offsets from its first transform
+  const size_t esz = f64 ? 16 : 8;
+  const long long lim = 1LL << 31;
+  if (l.ld == LD_PLAIN) {
+    const long long sp = tile_span_bytes(l.P.in, l.P.G, e->tpb, l.P.Lfft, esz);
+    if (sp < 0 || sp >= lim) return;
+  }
+  if (l.st == ST_PLAIN || l.st == ST_TW) {
+    const long long sp = tile_span_bytes(l.P.out, l.P.G, e->tpb, l.P.Lfft, esz);
+    if (sp < 0 || sp >= lim) return;
+  }
+  if (mode != 0 && (l.P.ke < 0 || (long long)l.P.Lfft * l.P.ke * (long long)esz >= lim)) return;
+  if (l.st == ST_TW) {
+    if (l.P.twsh != 0) return;
+    const long long top = ((long long)(l.P.Lfft - 1) * l.P.twk + (long long)(l.P.out.ncols - 1) * l.P.twc + 1) * esz;
+    if (l.P.twk < 0 || l.P.twc < 0 || top >= lim) return;
+  }
   l.kind = 3;
   l.sf = e;
   l.P.tpb = e->tpb;

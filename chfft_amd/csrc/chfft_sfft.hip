@@ -22,6 +22,8 @@
 // Reference correspondence: as for the generic kernels — the passes replace
 // mixed_radix.rs:104-368 / chirpz.rs:27-85, the operators the pre/post loops
 // of chirpz.rs, rfft1d.rs, dct1d.rs and mdct1d.rs.
+#include <type_traits>
+
 #include "chfft_device.h"
 
 namespace chfft {
@@ -56,9 +58,68 @@ constexpr int rad_twoff(Rad<RS...> r, int q) {
 }
 
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// Register budgets (bytes per thread) for twiddles fetched early: a pass's
+// table entries one pass ahead, and the four-step output twiddles of a
+// direct store together with the data.
+#ifndef CHFFT_TW_AHEAD_BYTES
+#define CHFFT_TW_AHEAD_BYTES 128
+#endif
+#ifndef CHFFT_TW_EARLY_BYTES
+#define CHFFT_TW_EARLY_BYTES 96
+#endif
 constexpr int lds_stride(int L, int tpb) {
   return (tpb > 1 && ((L + (L + 15) / 16) & 1) == 0) ? L + (L + 15) / 16 + 1 : L + (L + 15) / 16;
 }
+
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS accesses
+// (lgkmcnt) but leaves its global loads in flight, so that twiddles fetched
+// for a later pass keep arriving across the exchange. The "memory" clobber
+// keeps the compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Global access at a 32-bit byte offset from a workgroup-uniform base: the
+// backend then uses the scalar-base + vector-offset form of the global
+// instructions, with no 64-bit address arithmetic per element. The planner
+// checks that every tile's offsets fit (tile_span_bytes).
+template <typename V>
+__device__ __forceinline__ V ld_off(const void* base, unsigned off) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + off);
+}
+template <typename V>
+__device__ __forceinline__ void st_off(void* base, unsigned off, V v) {
+  *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
+// Padded LDS index of element base + j * S: linear in j when S is a multiple
+// of 16, so that the offsets fold into the instructions' immediates.
+template <int S>
+__device__ __forceinline__ int lxs(int base, int j) {
+  if constexpr (S % 16 == 0)
+    return lx(base) + j * (S + S / 16);
+  else
+    return lx(base + j * S);
+}
+
+// LDS layouts of a tile. Rows: transform t's element n at t * LP + lx(n)
+// (padded rows; lanes of one instruction walk one transform). Interleaved:
+// element n of transform t at n * TPB + t, for tiles whose consecutive lanes
+// take consecutive transforms: a lane group then covers whole bank rows
+// without padding, and every offset j * S of a butterfly folds into the
+// instruction's immediate.
+struct LdsRows {
+  template <int S>
+  __device__ __forceinline__ static int at(int base, int j) {
+    return lxs<S>(base, j);
+  }
+};
+template <int TPB>
+struct LdsInterleaved {
+  template <int S>
+  __device__ __forceinline__ static int at(int base, int j) {
+    return (base + j * S) * TPB;
+  }
+};
 
 template <class RS, int L, int TPT>
 struct Plan {
@@ -76,19 +137,42 @@ struct Plan {
 };
 
 // ---------------------------------------------------------------- pass pieces
+// Twiddles of pass q in registers, R-1 per butterfly group. When TPT is a
+// multiple of the span p, every butterfly of a thread has the same
+// k = i mod p and one group serves them all. k < p for any i, so the loads
+// need no guard and are issued all together, ahead of the butterflies.
 template <typename T, class PL, class RS, int q, int TPT>
-__device__ __forceinline__ void pass_compute(cx<T>* x, int u, const cx<T>* __restrict__ tw) {
-  constexpr int R = PL::R(q), p = PL::P(q), NB = PL::NB(q), B = PL::B(q);
-  constexpr int off = rad_twoff(RS{}, q);
+struct TwReg {
+  static constexpr int R = PL::R(q), p = PL::P(q), B = PL::B(q);
+  static constexpr int NG = p <= 1 ? 0 : (TPT % p == 0 ? 1 : B);
+  static constexpr int N = NG * (R - 1);
+  // small sets are fetched one pass early, before the previous pass computes
+  static constexpr bool AHEAD = N * (int)sizeof(cx<T>) <= CHFFT_TW_AHEAD_BYTES;
+  cx<T> w[N > 0 ? N : 1];
+  __device__ __forceinline__ void load(int u, const cx<T>* __restrict__ tw) {
+    constexpr int off = rad_twoff(RS{}, q);
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+      const unsigned k = (unsigned)(u + g * TPT) % (unsigned)p;
+#pragma unroll
+      for (int j = 1; j < R; j++)
+        w[g * (R - 1) + j - 1] = ld_off<cx<T>>(tw + off + (j - 1) * p, k * (unsigned)sizeof(cx<T>));
+    }
+  }
+};
+
+template <typename T, class PL, class RS, int q, int TPT>
+__device__ __forceinline__ void pass_compute(cx<T>* x, int u, const TwReg<T, PL, RS, q, TPT>& W) {
+  using TR = TwReg<T, PL, RS, q, TPT>;
+  constexpr int R = PL::R(q), NB = PL::NB(q), B = PL::B(q);
 #pragma unroll
   for (int b = 0; b < B; b++) {
     const int i = u + b * TPT;
     if (PL::FULL(q) || i < NB) {
-      if constexpr (p > 1) {
-        const int k = i % p;
-        const cx<T>* tq = tw + off + k;
+      if constexpr (TR::NG > 0) {
+        constexpr int gs = TR::NG == 1 ? 0 : 1;
 #pragma unroll
-        for (int j = 1; j < R; j++) x[b * R + j] = x[b * R + j] * tq[(j - 1) * p];
+        for (int j = 1; j < R; j++) x[b * R + j] = x[b * R + j] * W.w[gs * b * (R - 1) + j - 1];
       }
       dft<T, R>(x + b * R);
     }
@@ -96,7 +180,7 @@ __device__ __forceinline__ void pass_compute(cx<T>* x, int u, const cx<T>* __res
 }
 
 // Write the outputs of pass q at their Stockham positions.
-template <typename T, class PL, int q, int TPT>
+template <typename T, class PL, int q, int TPT, class LY>
 __device__ __forceinline__ void pass_write(const cx<T>* x, int u, cx<T>* row) {
   constexpr int R = PL::R(q), p = PL::P(q), NB = PL::NB(q), B = PL::B(q);
 #pragma unroll
@@ -106,13 +190,13 @@ __device__ __forceinline__ void pass_write(const cx<T>* x, int u, cx<T>* row) {
       const int k = i % p;
       const int o = (i - k) * R + k;
 #pragma unroll
-      for (int j = 0; j < R; j++) row[lx(o + j * p)] = x[b * R + j];
+      for (int j = 0; j < R; j++) row[LY::template at<p>(o, j)] = x[b * R + j];
     }
   }
 }
 
 // Read the inputs of pass q (element i + j*NB of butterfly i).
-template <typename T, class PL, int q, int TPT>
+template <typename T, class PL, int q, int TPT, class LY>
 __device__ __forceinline__ void pass_read(cx<T>* x, int u, const cx<T>* row) {
   constexpr int R = PL::R(q), NB = PL::NB(q), B = PL::B(q);
 #pragma unroll
@@ -120,33 +204,49 @@ __device__ __forceinline__ void pass_read(cx<T>* x, int u, const cx<T>* row) {
     const int i = u + b * TPT;
     if (PL::FULL(q) || i < NB) {
 #pragma unroll
-      for (int j = 0; j < R; j++) x[b * R + j] = row[lx(i + j * NB)];
+      for (int j = 0; j < R; j++) x[b * R + j] = row[LY::template at<NB>(i, j)];
     }
   }
 }
 
 // Exchange from pass qa's outputs to pass qb's inputs through LDS.
-template <typename T, class PL, int qa, int qb, int TPT, bool PP>
+template <typename T, class PL, int qa, int qb, int TPT, bool PP, class LY>
 __device__ __forceinline__ void exchange(cx<T>* x, int u, cx<T>* row0, cx<T>* row1, int& cur) {
   cx<T>* row = (PP && cur) ? row1 : row0;
-  pass_write<T, PL, qa, TPT>(x, u, row);
-  __syncthreads();
-  pass_read<T, PL, qb, TPT>(x, u, row);
+  pass_write<T, PL, qa, TPT, LY>(x, u, row);
+  lds_barrier();
+  pass_read<T, PL, qb, TPT, LY>(x, u, row);
   if constexpr (PP) {
     cur ^= 1;
   } else {
-    __syncthreads();
+    lds_barrier();
   }
 }
 
-template <typename T, class PL, class RS, int q, int TPT, bool PP>
+// Passes q.. of the sequence; W holds pass q's twiddles. Pass q+1's
+// twiddles are fetched before pass q computes (small sets) or right after,
+// so that their latency overlaps the exchange either way.
+template <typename T, class PL, class RS, int q, int TPT, bool PP, class LY>
 __device__ __forceinline__ void run_seq(cx<T>* x, int u, const cx<T>* __restrict__ tw, cx<T>* row0, cx<T>* row1,
-                                        int& cur) {
-  pass_compute<T, PL, RS, q, TPT>(x, u, tw);
+                                        int& cur, const TwReg<T, PL, RS, q, TPT>& W) {
   if constexpr (q + 1 < PL::NP) {
-    exchange<T, PL, q, q + 1, TPT, PP>(x, u, row0, row1, cur);
-    run_seq<T, PL, RS, q + 1, TPT, PP>(x, u, tw, row0, row1, cur);
+    using TN = TwReg<T, PL, RS, q + 1, TPT>;
+    TN Wn;
+    if constexpr (TN::AHEAD) Wn.load(u, tw);
+    pass_compute<T, PL, RS, q, TPT>(x, u, W);
+    if constexpr (!TN::AHEAD) Wn.load(u, tw);
+    exchange<T, PL, q, q + 1, TPT, PP, LY>(x, u, row0, row1, cur);
+    run_seq<T, PL, RS, q + 1, TPT, PP, LY>(x, u, tw, row0, row1, cur, Wn);
+  } else {
+    pass_compute<T, PL, RS, q, TPT>(x, u, W);
   }
+}
+
+template <typename T, class PL, class RS, int TPT, bool PP, class LY>
+__device__ __forceinline__ void run_all(cx<T>* x, int u, const cx<T>* __restrict__ tw, cx<T>* row0, cx<T>* row1,
+                                        int& cur) {
+  const TwReg<T, PL, RS, 0, TPT> W0{};  // the first pass has span 1: no twiddles
+  run_seq<T, PL, RS, 0, TPT, PP, LY>(x, u, tw, row0, row1, cur, W0);
 }
 
 // ---------------------------------------------------------------- store operators (element-wise)
@@ -200,33 +300,82 @@ __device__ __forceinline__ void store_elem(const KParams& P, void* dst, long lon
 // MODE: 0 plain passes; 1 fused Bluestein (chirp on load, spectrum product,
 // second pass sequence, de-chirp); 2 spectrum product + second sequence only
 // (the row pass of a four-step Bluestein).
-template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS>
-__device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, void* dst, int g0, double scale,
-                                          cx<T>* lds) {
-  constexpr bool BLUE = MODE == 1;
+// Natural-order store of a tile from its final LDS image (epilogue path) for
+// the element-wise operators PLAIN and TW; CM: consecutive threads on
+// consecutive transforms (transposed stores write whole segments).
+template <typename T, int L, int TPT, int TPB, int ST, bool CM>
+__device__ __forceinline__ void epi_store(const KParams& P, void* dst, const cx<T>* lds, int LP, int g0, int nt,
+                                          T s) {
+  const int tid = threadIdx.x;
+  const int t = CM ? tid % TPB : tid / TPT;
+  const int u = CM ? tid / TPB : tid % TPT;
+  if (t >= nt) return;
+  const cx<T>* row = lds + t * LP;
+  constexpr unsigned ESZ = sizeof(cx<T>);
+  int c0, c;
+  const long long o0 = io_base(P.out, g0, &c0);
+  const long long og = io_base(P.out, g0 + t, &c);
+  char* ob = reinterpret_cast<char*>(dst) + o0 * (long long)ESZ;
+  const unsigned es = (unsigned)P.out.estride * ESZ;
+  const unsigned voff = (unsigned)(og - o0) * ESZ + (unsigned)u * es;
+  const unsigned tk = (unsigned)P.twk * ESZ;
+  const unsigned toff = ((unsigned)u * (unsigned)P.twk + (unsigned)c * (unsigned)P.twc) * ESZ;
+  const bool cjo = P.conj_out != 0;
+  constexpr int S = cdiv(L, TPT);
+#pragma unroll
+  for (int r = 0; r < S; r++) {
+    if (L % TPT == 0 || u + r * TPT < L) {
+      cx<T> v = row[lxs<TPT>(u, r)];
+      if constexpr (ST == ST_TW) v = v * ld_off<cx<T>>(reinterpret_cast<const char*>(P.tab_a) + (unsigned)(r * TPT) * tk, toff);
+      if (cjo) v = cj(v);
+      st_off<cx<T>>(ob + (unsigned)(r * TPT) * es, voff, scl(v, s));
+    }
+  }
+}
+
+// First-pass gather of one tile: element i + j*L/R0 of butterfly i = u + b*TPT
+// of transform g0 + t.
+template <typename T, int L, int TPT, int TPB, bool COL, int LD, int MODE, class RS>
+__device__ __forceinline__ void sfft_gather(const KParams& P, const void* src, int g0,
+                                            cx<T> (&x)[Plan<RS, L, TPT>::EMAX()]) {
   using PL = Plan<RS, L, TPT>;
-  constexpr int NP = PL::NP;
-  constexpr int E = PL::EMAX();
-  constexpr int LP = lds_stride(L, TPB);
-  // a second LDS buffer saves one barrier per exchange but halves the
-  // workgroups that fit on a CU; only tiny tiles take it
-  constexpr bool PP = 2ull * TPB * LP * sizeof(cx<T>) <= 16384;
-  static_assert(TPB * LP * sizeof(cx<T>) <= kLdsBytes, "transform tile does not fit in LDS");
+  constexpr bool BLUE = MODE == 1;
+  constexpr unsigned ESZ = sizeof(cx<T>);
   const int tid = threadIdx.x;
   const int t = COL ? tid % TPB : tid / TPT;
   const int u = COL ? tid / TPB : tid % TPT;
   const int g = g0 + t;
   const bool valid = g < P.G;
-  cx<T>* row0 = lds + t * LP;
-  cx<T>* row1 = PP ? lds + (TPB + t) * LP : row0;
-  const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.seq.tw);
   const cx<T>* chirp = reinterpret_cast<const cx<T>*>(P.chirp);
-  int cur = 0;
-
-  cx<T> x[E];
-  // first pass gathered from global memory through the load operator
-  {
-    constexpr int R0 = PL::R(0), NB0 = PL::NB(0), B0 = PL::B(0);
+  constexpr int R0 = PL::R(0), NB0 = PL::NB(0), B0 = PL::B(0);
+  if constexpr (LD == LD_PLAIN) {
+    // threads past the last transform re-read the tile's first one (their
+    // results are never stored), so every load stays in bounds unguarded
+    int c0, c;
+    const long long b0 = io_base(P.in, g0, &c0);
+    const long long bg = io_base(P.in, valid ? g : g0, &c);
+    const char* tb = reinterpret_cast<const char*>(src) + b0 * (long long)ESZ;
+    const unsigned es = (unsigned)P.in.estride * ESZ;
+    const unsigned voff = (unsigned)(bg - b0) * ESZ + (unsigned)u * es;
+    const bool cji = P.conj_in != 0;
+#pragma unroll
+    for (int b = 0; b < B0; b++) {
+      const int i = u + b * TPT;
+      if (PL::FULL(0) || i < NB0) {
+#pragma unroll
+        for (int j = 0; j < R0; j++) {
+          const int rel = b * TPT + j * NB0;  // element n = u + rel
+          cx<T> v = mk(T(0), T(0));
+          if (!BLUE || u + rel < P.L) {
+            v = ld_off<cx<T>>(tb + (unsigned)rel * es, voff);
+            if (cji) v = cj(v);
+            if constexpr (BLUE) v = v * ld_off<cx<T>>(chirp + rel, (unsigned)u * ESZ);
+          }
+          x[b * R0 + j] = v;
+        }
+      }
+    }
+  } else {
     const Loader<T, LD> ld = make_loader<T, LD>(P, src, valid ? g : 0);
 #pragma unroll
     for (int b = 0; b < B0; b++) {
@@ -248,22 +397,84 @@ __device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, voi
       }
     }
   }
-  run_seq<T, PL, RS, 0, TPT, PP>(x, u, tw, row0, row1, cur);
+}
+
+// MODE: 0 plain passes; 1 fused Bluestein (chirp on load, spectrum product,
+// second pass sequence, de-chirp); 2 spectrum product + second sequence only
+// (the row pass of a four-step Bluestein).
+template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS>
+__device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, void* dst, int g0, double scale,
+                                          cx<T>* lds) {
+  constexpr bool BLUE = MODE == 1;
+  using PL = Plan<RS, L, TPT>;
+  constexpr int NP = PL::NP;
+  constexpr int E = PL::EMAX();
+  constexpr int LP = lds_stride(L, TPB);
+  constexpr unsigned ESZ = sizeof(cx<T>);
+  // a second LDS buffer saves one barrier per exchange but halves the
+  // workgroups that fit on a CU; only tiny tiles take it
+  constexpr bool PP = 2ull * TPB * LP * sizeof(cx<T>) <= 16384;
+  static_assert(TPB * LP * sizeof(cx<T>) <= kLdsBytes, "transform tile does not fit in LDS");
+  const int tid = threadIdx.x;
+  const int t = COL ? tid % TPB : tid / TPT;
+  const int u = COL ? tid / TPB : tid % TPT;
+  const int g = g0 + t;
+  const bool valid = g < P.G;
+  // the epilogue reads the final image as padded rows
+  constexpr bool IL = COL && !EPI;
+  using LY = std::conditional_t<IL, LdsInterleaved<TPB>, LdsRows>;
+  cx<T>* row0 = IL ? lds + t : lds + t * LP;
+  cx<T>* row1 = PP ? (IL ? lds + TPB * LP + t : lds + (TPB + t) * LP) : row0;
+  const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.seq.tw);
+  const cx<T>* chirp = reinterpret_cast<const cx<T>*>(P.chirp);
+  int cur = 0;
 
   constexpr int RL = PL::R(NP - 1), PLAST = PL::P(NP - 1), BL = PL::B(NP - 1);
+  cx<T> x[E];
+  sfft_gather<T, L, TPT, TPB, COL, LD, MODE, RS>(P, src, g0, x);
+
+  // four-step twiddles of the direct store, W^(k*twk + c*twc) for the
+  // outputs k = u + rel: fetched with the data when they fit a small
+  // register budget (their latency then hides behind the passes), else at
+  // the store
+  constexpr bool TW_EARLY = ST == ST_TW && !EPI && BL * RL * sizeof(cx<T>) <= CHFFT_TW_EARLY_BYTES;
+  cx<T> wst[TW_EARLY ? BL * RL : 1];
+  int c0o = 0, co = 0;
+  long long o0 = 0, og = 0;
+  if constexpr (ST == ST_PLAIN || ST == ST_TW) {
+    o0 = io_base(P.out, g0, &c0o);
+    og = io_base(P.out, valid ? g : g0, &co);
+  }
+  const unsigned tk = (unsigned)P.twk * ESZ;
+  const unsigned toff = ((unsigned)u * (unsigned)P.twk + (unsigned)co * (unsigned)P.twc) * ESZ;
+  if constexpr (TW_EARLY) {
+#pragma unroll
+    for (int b = 0; b < BL; b++) {
+      if (PL::FULL(NP - 1) || u + b * TPT < PL::NB(NP - 1)) {  // outputs k < L only
+#pragma unroll
+        for (int j = 0; j < RL; j++)
+          wst[b * RL + j] = ld_off<cx<T>>(
+              reinterpret_cast<const char*>(P.tab_a) + (unsigned)(b * TPT + j * PLAST) * tk, toff);
+      }
+    }
+  }
+  run_all<T, PL, RS, TPT, PP, LY>(x, u, tw, row0, row1, cur);
+
   if constexpr (MODE != 0) {
     // spectrum product at the last-pass positions, conjugate, second sequence
-    const cx<T>* K = reinterpret_cast<const cx<T>*>(P.kspec) + (long long)(g % P.in.ncols) * P.kc;
+    const cx<T>* K = reinterpret_cast<const cx<T>*>(P.kspec) + (long long)((valid ? g : g0) % P.in.ncols) * P.kc;
+    const unsigned ke = (unsigned)P.ke * ESZ;
 #pragma unroll
     for (int b = 0; b < BL; b++) {
       const int i = u + b * TPT;
       if (PL::FULL(NP - 1) || i < PL::NB(NP - 1)) {
 #pragma unroll
-        for (int j = 0; j < RL; j++) x[b * RL + j] = cj(x[b * RL + j] * K[(long long)(i + j * PLAST) * P.ke]);
+        for (int j = 0; j < RL; j++)
+          x[b * RL + j] = cj(x[b * RL + j] * ld_off<cx<T>>(K, (unsigned)(i + j * PLAST) * ke));
       }
     }
-    exchange<T, PL, NP - 1, 0, TPT, PP>(x, u, row0, row1, cur);
-    run_seq<T, PL, RS, 0, TPT, PP>(x, u, tw, row0, row1, cur);
+    exchange<T, PL, NP - 1, 0, TPT, PP, LY>(x, u, row0, row1, cur);
+    run_all<T, PL, RS, TPT, PP, LY>(x, u, tw, row0, row1, cur);
   }
   if constexpr (BLUE) {
 #pragma unroll
@@ -272,45 +483,80 @@ __device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, voi
       if (PL::FULL(NP - 1) || i < PL::NB(NP - 1)) {
 #pragma unroll
         for (int j = 0; j < RL; j++) {
-          const int k = i + j * PLAST;
-          if (k < P.L) x[b * RL + j] = cj(x[b * RL + j]) * chirp[k];
+          const int rel = b * TPT + j * PLAST;  // k = u + rel
+          if (u + rel < P.L) x[b * RL + j] = cj(x[b * RL + j]) * ld_off<cx<T>>(chirp + rel, (unsigned)u * ESZ);
         }
       }
     }
   }
 
   if constexpr (EPI) {
-    // operators that pair bins: through LDS in natural order
+    // through LDS in natural order
     cx<T>* row = (PP && cur) ? row1 : row0;
 #pragma unroll
     for (int b = 0; b < BL; b++) {
       const int i = u + b * TPT;
       if (PL::FULL(NP - 1) || i < PL::NB(NP - 1)) {
 #pragma unroll
-        for (int j = 0; j < RL; j++) row[lx(i + j * PLAST)] = x[b * RL + j];
+        for (int j = 0; j < RL; j++) row[LY::template at<PLAST>(i, j)] = x[b * RL + j];
       }
     }
-    __syncthreads();
-    // the planner sets P.Lp == LP for specialised launches
-    const TMap m = make_map(TPB);
-    store_stage<T, ST>(P, m, dst, (PP && cur) ? lds + TPB * LP : lds, g0, min(TPB, P.G - g0), scale);
+    lds_barrier();
+    const cx<T>* img = (PP && cur) ? lds + TPB * LP : lds;
+    const int nt = min(TPB, P.G - g0);
+    if constexpr (ST == ST_PLAIN || ST == ST_TW) {
+      if (P.colmajor_st)
+        epi_store<T, L, TPT, TPB, ST, true>(P, dst, img, LP, g0, nt, (T)scale);
+      else
+        epi_store<T, L, TPT, TPB, ST, false>(P, dst, img, LP, g0, nt, (T)scale);
+    } else {
+      // the planner sets P.Lp == LP for specialised launches
+      const TMap m = make_map(TPB);
+      store_stage<T, ST>(P, m, dst, img, g0, nt, scale);
+    }
   } else {
     if (!valid) return;
-    int c = 0;
-    long long obase;
-    if constexpr (ST == ST_PLAIN || ST == ST_TW || ST == ST_FS_DECHIRP)
-      obase = io_base(P.out, g, &c);
-    else
-      obase = (long long)g * P.out.bdist;
     const T s = (T)scale;
+    if constexpr (ST == ST_PLAIN || ST == ST_TW) {
+      char* ob = reinterpret_cast<char*>(dst) + o0 * (long long)ESZ;
+      const unsigned es = (unsigned)P.out.estride * ESZ;
+      const unsigned voff = (unsigned)(og - o0) * ESZ + (unsigned)u * es;
+      const bool cjo = P.conj_out != 0;
 #pragma unroll
-    for (int b = 0; b < BL; b++) {
-      const int i = u + b * TPT;
-      if (PL::FULL(NP - 1) || i < PL::NB(NP - 1)) {
+      for (int b = 0; b < BL; b++) {
+        const int i = u + b * TPT;
+        if (PL::FULL(NP - 1) || i < PL::NB(NP - 1)) {
 #pragma unroll
-        for (int j = 0; j < RL; j++) {
-          const int k = i + j * PLAST;
-          if (!BLUE || k < P.L) store_elem<T, ST>(P, dst, obase, c, k, x[b * RL + j], s);
+          for (int j = 0; j < RL; j++) {
+            const int rel = b * TPT + j * PLAST;  // k = u + rel
+            if (!BLUE || u + rel < P.L) {
+              cx<T> v = x[b * RL + j];
+              if constexpr (TW_EARLY)
+                v = v * wst[b * RL + j];
+              else if constexpr (ST == ST_TW)
+                v = v * ld_off<cx<T>>(reinterpret_cast<const char*>(P.tab_a) + (unsigned)rel * tk, toff);
+              if (cjo) v = cj(v);
+              st_off<cx<T>>(ob + (unsigned)rel * es, voff, scl(v, s));
+            }
+          }
+        }
+      }
+    } else {
+      int c = 0;
+      long long obase;
+      if constexpr (ST == ST_FS_DECHIRP)
+        obase = io_base(P.out, g, &c);
+      else
+        obase = (long long)g * P.out.bdist;
+#pragma unroll
+      for (int b = 0; b < BL; b++) {
+        const int i = u + b * TPT;
+        if (PL::FULL(NP - 1) || i < PL::NB(NP - 1)) {
+#pragma unroll
+          for (int j = 0; j < RL; j++) {
+            const int k = i + j * PLAST;
+            if (!BLUE || k < P.L) store_elem<T, ST>(P, dst, obase, c, k, x[b * RL + j], s);
+          }
         }
       }
     }
