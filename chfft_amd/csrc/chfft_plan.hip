@@ -429,6 +429,13 @@ void fill_inner(KParams& P, const Inner& in) {
   P.ke = 1;
 }
 
+int device_cus() {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) return 256;
+  return n;
+}
+
 // Largest byte offset of one tile's accesses (tpb consecutive transforms of
 // len elements under d) from the tile's first transform; -1 when an offset
 // could be negative (transform bases not increasing with g).
@@ -708,12 +715,7 @@ struct C2CEngine {
     return false;
   }
 
-  static int num_cus() {
-    int dev = 0, n = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 256;
-    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) return 256;
-    return n;
-  }
+  static int num_cus() { return device_cus(); }
 
   int fourstep_grid(size_t batch) const {
     // phases share one workgroup size and LDS size (set in emit_persistent)

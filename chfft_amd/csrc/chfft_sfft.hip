@@ -68,8 +68,21 @@ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 #ifndef CHFFT_TW_EARLY_BYTES
 #define CHFFT_TW_EARLY_BYTES 96
 #endif
+// Per-pass twiddle tables up to this size are copied into LDS once per tile
+// and read from there (shorter latency than the L2, and off the vector
+// memory path).
+#ifndef CHFFT_TW_LDS_BYTES
+#define CHFFT_TW_LDS_BYTES 2048
+#endif
+constexpr bool tw_in_lds(int entries, int esz) { return entries > 0 && entries * esz <= CHFFT_TW_LDS_BYTES; }
 constexpr int lds_stride(int L, int tpb) {
   return (tpb > 1 && ((L + (L + 15) / 16) & 1) == 0) ? L + (L + 15) / 16 + 1 : L + (L + 15) / 16;
+}
+// LDS bytes of one tile: one padded buffer, or two (ping-pong) when both
+// fit in 16 KB, plus the per-pass twiddle tables when they are kept in LDS.
+constexpr size_t lds_need(int L, int tpb, int esz, int ntw) {
+  const size_t one = (size_t)tpb * lds_stride(L, tpb) * esz;
+  return (2 * one <= 16384 ? 2 * one : one) + (tw_in_lds(ntw, esz) ? (size_t)ntw * esz : 0);
 }
 
 // Workgroup barrier for LDS traffic only: waits for this wave's LDS accesses
@@ -149,14 +162,20 @@ struct TwReg {
   // small sets are fetched one pass early, before the previous pass computes
   static constexpr bool AHEAD = N * (int)sizeof(cx<T>) <= CHFFT_TW_AHEAD_BYTES;
   cx<T> w[N > 0 ? N : 1];
+  // LDT: tw is the table's copy in LDS
+  template <bool LDT>
   __device__ __forceinline__ void load(int u, const cx<T>* __restrict__ tw) {
     constexpr int off = rad_twoff(RS{}, q);
 #pragma unroll
     for (int g = 0; g < NG; g++) {
       const unsigned k = (unsigned)(u + g * TPT) % (unsigned)p;
 #pragma unroll
-      for (int j = 1; j < R; j++)
-        w[g * (R - 1) + j - 1] = ld_off<cx<T>>(tw + off + (j - 1) * p, k * (unsigned)sizeof(cx<T>));
+      for (int j = 1; j < R; j++) {
+        if constexpr (LDT)
+          w[g * (R - 1) + j - 1] = tw[off + (j - 1) * p + k];
+        else
+          w[g * (R - 1) + j - 1] = ld_off<cx<T>>(tw + off + (j - 1) * p, k * (unsigned)sizeof(cx<T>));
+      }
     }
   }
 };
@@ -223,30 +242,33 @@ __device__ __forceinline__ void exchange(cx<T>* x, int u, cx<T>* row0, cx<T>* ro
   }
 }
 
-// Passes q.. of the sequence; W holds pass q's twiddles. Pass q+1's
-// twiddles are fetched before pass q computes (small sets) or right after,
-// so that their latency overlaps the exchange either way.
-template <typename T, class PL, class RS, int q, int TPT, bool PP, class LY>
+// Passes q.. of the sequence; W holds pass q's twiddles. From global
+// memory, pass q+1's twiddles are fetched before pass q computes (small
+// sets) or right after, so that their latency overlaps the exchange; from
+// the LDS copy (LDT) they are read after the exchange's barrier, which also
+// publishes the copy.
+template <typename T, class PL, class RS, int q, int TPT, bool PP, class LY, bool LDT>
 __device__ __forceinline__ void run_seq(cx<T>* x, int u, const cx<T>* __restrict__ tw, cx<T>* row0, cx<T>* row1,
                                         int& cur, const TwReg<T, PL, RS, q, TPT>& W) {
   if constexpr (q + 1 < PL::NP) {
     using TN = TwReg<T, PL, RS, q + 1, TPT>;
     TN Wn;
-    if constexpr (TN::AHEAD) Wn.load(u, tw);
+    if constexpr (!LDT && TN::AHEAD) Wn.template load<false>(u, tw);
     pass_compute<T, PL, RS, q, TPT>(x, u, W);
-    if constexpr (!TN::AHEAD) Wn.load(u, tw);
+    if constexpr (!LDT && !TN::AHEAD) Wn.template load<false>(u, tw);
     exchange<T, PL, q, q + 1, TPT, PP, LY>(x, u, row0, row1, cur);
-    run_seq<T, PL, RS, q + 1, TPT, PP, LY>(x, u, tw, row0, row1, cur, Wn);
+    if constexpr (LDT) Wn.template load<true>(u, tw);
+    run_seq<T, PL, RS, q + 1, TPT, PP, LY, LDT>(x, u, tw, row0, row1, cur, Wn);
   } else {
     pass_compute<T, PL, RS, q, TPT>(x, u, W);
   }
 }
 
-template <typename T, class PL, class RS, int TPT, bool PP, class LY>
+template <typename T, class PL, class RS, int TPT, bool PP, class LY, bool LDT>
 __device__ __forceinline__ void run_all(cx<T>* x, int u, const cx<T>* __restrict__ tw, cx<T>* row0, cx<T>* row1,
                                         int& cur) {
   const TwReg<T, PL, RS, 0, TPT> W0{};  // the first pass has span 1: no twiddles
-  run_seq<T, PL, RS, 0, TPT, PP, LY>(x, u, tw, row0, row1, cur, W0);
+  run_seq<T, PL, RS, 0, TPT, PP, LY, LDT>(x, u, tw, row0, row1, cur, W0);
 }
 
 // ---------------------------------------------------------------- store operators (element-wise)
@@ -399,12 +421,39 @@ __device__ __forceinline__ void sfft_gather(const KParams& P, const void* src, i
   }
 }
 
+// Four-step twiddles of a direct store, W^(k*twk + c*twc) for the outputs
+// k = u + rel of a thread whose transform is column c: fetched with the data
+// when they fit a small register budget (their latency then hides behind
+// the passes), else read at the store.
+template <typename T, class PL, int TPT, int ST, bool EPI>
+struct StoreTw {
+  static constexpr int NP = PL::NP, RL = PL::R(NP - 1), PLAST = PL::P(NP - 1), BL = PL::B(NP - 1);
+  static constexpr bool EARLY = ST == ST_TW && !EPI && BL * RL * sizeof(cx<T>) <= CHFFT_TW_EARLY_BYTES;
+  cx<T> w[EARLY ? BL * RL : 1];
+  __device__ __forceinline__ void load(const KParams& P, int u, int c) {
+    if constexpr (EARLY) {
+      constexpr unsigned ESZ = sizeof(cx<T>);
+      const unsigned tk = (unsigned)P.twk * ESZ;
+      const unsigned toff = ((unsigned)u * (unsigned)P.twk + (unsigned)c * (unsigned)P.twc) * ESZ;
+#pragma unroll
+      for (int b = 0; b < BL; b++) {
+        if (PL::FULL(NP - 1) || u + b * TPT < PL::NB(NP - 1)) {  // outputs k < L only
+#pragma unroll
+          for (int j = 0; j < RL; j++)
+            w[b * RL + j] =
+                ld_off<cx<T>>(reinterpret_cast<const char*>(P.tab_a) + (unsigned)(b * TPT + j * PLAST) * tk, toff);
+        }
+      }
+    }
+  }
+};
+
 // MODE: 0 plain passes; 1 fused Bluestein (chirp on load, spectrum product,
 // second pass sequence, de-chirp); 2 spectrum product + second sequence only
 // (the row pass of a four-step Bluestein).
 template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS>
-__device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, void* dst, int g0, double scale,
-                                          cx<T>* lds) {
+__device__ __forceinline__ void sfft_body(const KParams& P, const void* src, void* dst, int g0, double scale,
+                                          cx<T>* lds, const StoreTw<T, Plan<RS, L, TPT>, TPT, ST, EPI>& stw) {
   constexpr bool BLUE = MODE == 1;
   using PL = Plan<RS, L, TPT>;
   constexpr int NP = PL::NP;
@@ -414,7 +463,10 @@ __device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, voi
   // a second LDS buffer saves one barrier per exchange but halves the
   // workgroups that fit on a CU; only tiny tiles take it
   constexpr bool PP = 2ull * TPB * LP * sizeof(cx<T>) <= 16384;
-  static_assert(TPB * LP * sizeof(cx<T>) <= kLdsBytes, "transform tile does not fit in LDS");
+  constexpr int NTW = rad_twoff(RS{}, NP);
+  constexpr bool LDT = tw_in_lds(NTW, (int)sizeof(cx<T>));
+  static_assert(((PP ? 2 : 1) * TPB * LP + (LDT ? NTW : 0)) * sizeof(cx<T>) <= kLdsBytes,
+                "transform tile does not fit in LDS");
   const int tid = threadIdx.x;
   const int t = COL ? tid % TPB : tid / TPT;
   const int u = COL ? tid / TPB : tid % TPT;
@@ -432,13 +484,17 @@ __device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, voi
   constexpr int RL = PL::R(NP - 1), PLAST = PL::P(NP - 1), BL = PL::B(NP - 1);
   cx<T> x[E];
   sfft_gather<T, L, TPT, TPB, COL, LD, MODE, RS>(P, src, g0, x);
-
-  // four-step twiddles of the direct store, W^(k*twk + c*twc) for the
-  // outputs k = u + rel: fetched with the data when they fit a small
-  // register budget (their latency then hides behind the passes), else at
-  // the store
-  constexpr bool TW_EARLY = ST == ST_TW && !EPI && BL * RL * sizeof(cx<T>) <= CHFFT_TW_EARLY_BYTES;
-  cx<T> wst[TW_EARLY ? BL * RL : 1];
+  // per-pass twiddle tables copied into LDS behind the tile buffers
+  const cx<T>* twp = tw;
+  if constexpr (LDT) {
+    cx<T>* twl = lds + (PP ? 2 : 1) * TPB * LP;
+    constexpr int TH = TPT * TPB;
+#pragma unroll
+    for (int e0 = 0; e0 < NTW; e0 += TH)
+      if (e0 + tid < NTW) twl[e0 + tid] = ld_off<cx<T>>(tw + e0, (unsigned)tid * ESZ);
+    twp = twl;
+  }
+  constexpr bool TW_EARLY = StoreTw<T, PL, TPT, ST, EPI>::EARLY;
   int c0o = 0, co = 0;
   long long o0 = 0, og = 0;
   if constexpr (ST == ST_PLAIN || ST == ST_TW) {
@@ -447,18 +503,7 @@ __device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, voi
   }
   const unsigned tk = (unsigned)P.twk * ESZ;
   const unsigned toff = ((unsigned)u * (unsigned)P.twk + (unsigned)co * (unsigned)P.twc) * ESZ;
-  if constexpr (TW_EARLY) {
-#pragma unroll
-    for (int b = 0; b < BL; b++) {
-      if (PL::FULL(NP - 1) || u + b * TPT < PL::NB(NP - 1)) {  // outputs k < L only
-#pragma unroll
-        for (int j = 0; j < RL; j++)
-          wst[b * RL + j] = ld_off<cx<T>>(
-              reinterpret_cast<const char*>(P.tab_a) + (unsigned)(b * TPT + j * PLAST) * tk, toff);
-      }
-    }
-  }
-  run_all<T, PL, RS, TPT, PP, LY>(x, u, tw, row0, row1, cur);
+  run_all<T, PL, RS, TPT, PP, LY, LDT>(x, u, twp, row0, row1, cur);
 
   if constexpr (MODE != 0) {
     // spectrum product at the last-pass positions, conjugate, second sequence
@@ -474,7 +519,7 @@ __device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, voi
       }
     }
     exchange<T, PL, NP - 1, 0, TPT, PP, LY>(x, u, row0, row1, cur);
-    run_all<T, PL, RS, TPT, PP, LY>(x, u, tw, row0, row1, cur);
+    run_all<T, PL, RS, TPT, PP, LY, LDT>(x, u, twp, row0, row1, cur);
   }
   if constexpr (BLUE) {
 #pragma unroll
@@ -532,7 +577,7 @@ __device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, voi
             if (!BLUE || u + rel < P.L) {
               cx<T> v = x[b * RL + j];
               if constexpr (TW_EARLY)
-                v = v * wst[b * RL + j];
+                v = v * stw.w[b * RL + j];
               else if constexpr (ST == ST_TW)
                 v = v * ld_off<cx<T>>(reinterpret_cast<const char*>(P.tab_a) + (unsigned)rel * tk, toff);
               if (cjo) v = cj(v);
@@ -561,6 +606,21 @@ __device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, voi
       }
     }
   }
+}
+
+template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS>
+__device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, void* dst, int g0, double scale,
+                                          cx<T>* lds) {
+  StoreTw<T, Plan<RS, L, TPT>, TPT, ST, EPI> stw;
+  if constexpr (decltype(stw)::EARLY) {
+    const int tid = threadIdx.x;
+    const int t = COL ? tid % TPB : tid / TPT;
+    const int u = COL ? tid / TPB : tid % TPT;
+    int c;
+    io_base(P.out, min(g0 + t, P.G - 1), &c);
+    stw.load(P, u, c);
+  }
+  sfft_body<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, RS>(P, src, dst, g0, scale, lds, stw);
 }
 
 // Tile index of this workgroup, arranged so that the workgroups one XCD runs
@@ -705,22 +765,22 @@ struct SfsEntry {
   const void* fn;
 };
 
-template <int L, int TPB, bool F64>
+template <int L, int TPB, bool F64, class RS>
 constexpr size_t tile_bytes() {
-  return (size_t)TPB * lds_stride(L, TPB) * (F64 ? 16 : 8);
+  return lds_need(L, TPB, F64 ? 16 : 8, rad_twoff(RS{}, rad_count(RS{})));
 }
 constexpr size_t mx(size_t a, size_t b) { return a > b ? a : b; }
 
 #define SFS2(F64, T, N1, N2, TPT, TPB, RA, RB)                                                                  \
   {F64, N1, N2, 2, TPT * TPB, {TPB, TPB, 1}, {lds_stride(N1, TPB), lds_stride(N2, TPB), 0},                    \
-   mx(tile_bytes<N1, TPB, F64>(), tile_bytes<N2, TPB, F64>()),                                                  \
+   mx(tile_bytes<N1, TPB, F64, RA>(), tile_bytes<N2, TPB, F64, RB>()),                                          \
    reinterpret_cast<const void*>(                                                                               \
        &sfs_kernel<T, Phase<N1, TPT, TPB, true, LD_PLAIN, ST_TW, 0, false, RA>,                                \
                    Phase<N2, TPT, TPB, false, LD_PLAIN, ST_PLAIN, 0, true, RB>, NoPhase<TPT * TPB>>)}
 
 #define SFS3(F64, T, N1, N2, TPT, TPB, RA, RB)                                                                  \
   {F64, N1, N2, 3, TPT * TPB, {TPB, TPB, TPB}, {lds_stride(N1, TPB), lds_stride(N2, TPB), lds_stride(N1, TPB)}, \
-   mx(tile_bytes<N1, TPB, F64>(), tile_bytes<N2, TPB, F64>()),                                                  \
+   mx(tile_bytes<N1, TPB, F64, RA>(), tile_bytes<N2, TPB, F64, RB>()),                                          \
    reinterpret_cast<const void*>(                                                                               \
        &sfs_kernel<T, Phase<N1, TPT, TPB, true, LD_FS_CHIRP, ST_TW, 0, false, RA>,                             \
                    Phase<N2, TPT, TPB, false, LD_PLAIN, ST_TW, 2, false, RB>,                                  \
@@ -789,9 +849,12 @@ hipError_t sfft_set_lds_limits() {
 
 // LDS bytes an entry needs (two buffers when they fit, else one).
 size_t sfft_lds_bytes(const SfftEntry* e) {
-  const size_t esz = e->f64 ? 16 : 8;
-  const size_t one = (size_t)e->tpb * lds_stride(e->L, e->tpb) * esz;
-  return 2 * one <= 16384 ? 2 * one : one;  // same rule as the kernel's PP
+  int ntw = 0, p = 1;
+  for (int q = 0; q < e->nrad; q++) {
+    if (p > 1) ntw += (e->rad[q] - 1) * p;
+    p *= e->rad[q];
+  }
+  return lds_need(e->L, e->tpb, e->f64 ? 16 : 8, ntw);
 }
 
 int sfft_lds_stride(const SfftEntry* e) { return lds_stride(e->L, e->tpb); }

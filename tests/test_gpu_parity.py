@@ -264,3 +264,26 @@ def test_fourstep_schedules(monkeypatch, schedule, n, batch, dtype):
     for m in ("forward", "backward", "forwardu"):
         got = getattr(fft, m)(x)
         check(got, oracle.c2c(xr, m), f64, n)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_headline_full_batch(dtype):
+    # N = 43200 at the full batch of 512 on device buffers: column workgroups
+    # then walk runs of several transforms; spot-check transforms across the
+    # batch (first, both ends of interior runs, last) against the oracle
+    import torch
+    from chfft_amd import Plan
+    f64 = dtype == np.float64
+    cdt = np.complex128 if f64 else np.complex64
+    n, batch = 43200, 512
+    r = rng(43)
+    x = (r.standard_normal((batch, n)) + 1j * r.standard_normal((batch, n))).astype(cdt)
+    p = Plan("c2c", f64, batch, n)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    for m in ("forward", "backward"):
+        p.exec(xd, yd, m)
+        torch.cuda.synchronize()
+        y = yd.cpu().numpy()
+        for b in (0, 1, 7, 8, 255, 256, 510, 511):
+            check(y[b], oracle.c2c(x[b].astype(np.complex128), m), f64, n)
