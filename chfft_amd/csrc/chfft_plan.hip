@@ -12,8 +12,9 @@
 //   * longer non-7-smooth lengths      -> Bluestein over a four-step M-point
 //     FFT (3 launches: chirp + columns, rows + spectrum product + inverse rows,
 //     inverse columns + de-chirp).
-// The MAX_PRIME = 7 rule and the Bluestein padding M = 2 * next_pow2(N) follow
-// cfft1d.rs:104 and cfft1d.rs:114.
+// The MAX_PRIME = 7 rule follows cfft1d.rs:104. The fused Bluestein pads to
+// M = 2 * next_pow2(N) as cfft1d.rs:114 does; the four-step Bluestein takes
+// the cheapest smooth M >= 2N - 1 (bluestein_length).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -153,38 +154,58 @@ std::vector<ld_t> chirp_table(size_t N) {
   return v;
 }
 
-// In-place radix-2 FFT of a power-of-two length in double precision with
-// long-double twiddles (used once per Bluestein plan for the kernel spectrum).
-void host_fft_pow2(std::vector<double>& re, std::vector<double>& im) {
-  size_t n = re.size();
-  for (size_t i = 1, j = 0; i < n; i++) {
-    size_t bit = n >> 1;
-    for (; j & bit; bit >>= 1) j ^= bit;
-    j ^= bit;
-    if (i < j) {
-      std::swap(re[i], re[j]);
-      std::swap(im[i], im[j]);
-    }
-  }
-  std::vector<double> wr(n / 2 + 1), wi(n / 2 + 1);
-  for (size_t k = 0; k < n / 2; k++) {
+// Forward FFT of a 7-smooth length in double precision with long-double
+// twiddles (used once per Bluestein plan for the kernel spectrum): Stockham
+// passes of radix 4, 2, 3, 5, 7, each butterfly a direct DFT.
+void host_fft(std::vector<double>& re, std::vector<double>& im) {
+  const size_t n = re.size();
+  if (n <= 1) return;
+  std::vector<double> wr(n), wi(n);  // W_n^t, t < n
+  for (size_t t = 0; t < n; t++) {
     ld_t a, b;
-    unit_root(k, n, &a, &b);
-    wr[k] = (double)a;
-    wi[k] = (double)b;
+    unit_root(t, n, &a, &b);
+    wr[t] = (double)a;
+    wi[t] = (double)b;
   }
-  for (size_t len = 2; len <= n; len <<= 1) {
-    size_t step = n / len, half = len / 2;
-    for (size_t s = 0; s < n; s += len)
-      for (size_t k = 0; k < half; k++) {
-        double c = wr[k * step], d = wi[k * step];
-        size_t a = s + k, b = a + half;
-        double xr = re[b] * c - im[b] * d, xi = re[b] * d + im[b] * c;
-        re[b] = re[a] - xr;
-        im[b] = im[a] - xi;
-        re[a] += xr;
-        im[a] += xi;
+  std::vector<int> rad;
+  size_t m = n;
+  while (m % 4 == 0) {
+    rad.push_back(4);
+    m /= 4;
+  }
+  for (int q : {2, 3, 5, 7})
+    while (m % q == 0) {
+      rad.push_back(q);
+      m /= q;
+    }
+  std::vector<double> yr(n), yi(n);
+  size_t p = 1;
+  for (int r : rad) {
+    const size_t nb = n / r;
+    for (size_t i = 0; i < nb; i++) {
+      const size_t k = i % p;
+      double ar[8], ai[8];
+      for (int j = 0; j < r; j++) {  // inputs times W_{p r}^{j k} = W_n^{j k n / (p r)}
+        const size_t t = ((size_t)j * k * (n / (p * r))) % n;
+        const double xr = re[i + j * nb], xi = im[i + j * nb];
+        ar[j] = xr * wr[t] - xi * wi[t];
+        ai[j] = xr * wi[t] + xi * wr[t];
       }
+      for (int j = 0; j < r; j++) {  // DFT_r: W_r^{j m} = W_n^{j m n / r}
+        double sr = 0, si = 0;
+        for (int q = 0; q < r; q++) {
+          const size_t t = ((size_t)j * q % r) * (n / r);
+          sr += ar[q] * wr[t] - ai[q] * wi[t];
+          si += ar[q] * wi[t] + ai[q] * wr[t];
+        }
+        const size_t o = (i - k) * r + k + j * p;
+        yr[o] = sr;
+        yi[o] = si;
+      }
+    }
+    re.swap(yr);
+    im.swap(yi);
+    p *= r;
   }
 }
 
@@ -202,7 +223,7 @@ std::vector<ld_t> bluestein_spectrum(size_t N, size_t M) {
     re[i] = re[M - i];
     im[i] = im[M - i];
   }
-  host_fft_pow2(re, im);
+  host_fft(re, im);
   std::vector<ld_t> v(2 * M);
   for (size_t k = 0; k < M; k++) {
     v[2 * k] = (ld_t)re[k] / (ld_t)M;
@@ -395,6 +416,51 @@ std::vector<ld_t> fourstep_table(size_t n1, size_t n2) {
 }
 
 // Can a length-L inner transform be fused into one workgroup?
+// Four-step split total = n1 * n2 (n1 <= n2, both 7-smooth): rows of the
+// column pass 128-byte aligned (n2 a multiple of 16) if possible, then the
+// most balanced; 0 if there is none.
+size_t fourstep_split(size_t total, bool f64) {
+  (void)f64;
+  size_t best1 = 0;
+  bool best_aligned = false;
+  for (size_t d = 1; d * d <= total; d++) {
+    if (total % d != 0 || !smooth7(d) || !smooth7(total / d)) continue;
+    const bool aligned = (total / d) % 16 == 0;
+    if (aligned || !best_aligned) {
+      best1 = d;
+      best_aligned = aligned;
+    }
+  }
+  return best1;
+}
+
+// Padded length of a four-step Bluestein. The reference pads to
+// M = 2 * next_pow2(N) (cfft1d.rs:114); every M >= 2N - 1 holds the same
+// linear convolution of the chirped input with the chirp, so the results
+// agree to rounding. Taken here: the 5-smooth M in [2N - 1, 2 * next_pow2(N)]
+// with an aligned split (n2 % 16 == 0, n2 fusable) that needs the least work,
+// M * (passes of n1 + passes of n2) -- 20480 = 128 x 160 for N = 10007 and
+// 147456 = 384 x 384 for N = 65537 instead of 32768 and 262144.
+size_t bluestein_length(size_t N, bool f64) {
+  const size_t hi = 2 * next_pow2(N);
+  size_t best = hi;
+  double best_cost = 1e300;
+  for (size_t m = 2 * N - 1; m <= hi; m++) {
+    size_t r = m;
+    for (size_t q : {2, 3, 5})
+      while (r % q == 0) r /= q;
+    if (r != 1) continue;
+    const size_t d = fourstep_split(m, f64);
+    if (d == 0 || (m / d) % 16 != 0 || (m / d) * 4 > lds_cap(f64)) continue;
+    const double cost = (double)m * (double)(radices_for(d).size() + radices_for(m / d).size());
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = m;
+    }
+  }
+  return best;
+}
+
 bool fusable(size_t L, bool f64) {
   const size_t cap = lds_cap(f64);
   if (smooth7(L)) return L <= cap;
@@ -533,23 +599,10 @@ struct C2CEngine {
       mode = 1;
       return build_inner(len, f64, inner);
     }
-    size_t total = smooth7(len) ? len : 2 * next_pow2(len);
     mode = smooth7(len) ? 2 : 3;
+    const size_t total = mode == 2 ? len : bluestein_length(len, f64);
     M = total;
-    // most balanced split total = n1 * n2 with n1 <= n2, both fusable with
-    // several transforms per workgroup
-    // split total = n1 * n2 (n1 <= n2): rows of the column pass 128-byte
-    // aligned (n2 a multiple of 16) if possible, then the most balanced
-    size_t best1 = 0;
-    bool best_aligned = false;
-    for (size_t d = 1; d * d <= total; d++) {
-      if (total % d != 0 || !smooth7(d) || !smooth7(total / d)) continue;
-      const bool aligned = (total / d) % 16 == 0;
-      if (aligned || !best_aligned) {
-        best1 = d;
-        best_aligned = aligned;
-      }
-    }
+    const size_t best1 = fourstep_split(total, f64);
     if (best1 == 0) return CHFFT_ERR_UNSUPPORTED;
     n1 = best1;
     n2 = total / best1;
