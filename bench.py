@@ -41,7 +41,28 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--suite", action="store_true", help="also print one line per configuration")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only time a plain device-to-device copy of this many bytes (the reference kernel for "
+                         "the HBM traffic counters and the achievable copy bandwidth)")
     return ap.parse_args()
+
+
+def copy_calibration(torch, nbytes, reps=10):
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").uniform_()
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"copy_bytes": a.numel() * 4, "us": round(ms * 1e3, 2),
+            "gb_s": round(2 * a.numel() * 4 / (ms * 1e6), 1)}
 
 
 class Bufs:
@@ -64,21 +85,27 @@ class Bufs:
         return self.ins[j].data_ptr(), self.outs[j].data_ptr()
 
 
-def time_plan(torch, plan, bufs, steps, warmup, method="forward"):
+def time_plan(torch, plan, bufs, steps, warmup, method="forward", blocks=1):
+    """ms per step over `steps` steps bracketed by events; with blocks > 1
+    the median of that many such blocks (the per-configuration lines, whose
+    microsecond steps are sensitive to one-off host stalls)."""
     st = torch.cuda.current_stream().cuda_stream
     for i in range(warmup):
         a, b = bufs.pair(i)
         plan.exec_ptr(a, b, method, st)
     torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(torch.cuda.current_stream())
-    for i in range(steps):
-        a, b = bufs.pair(i)
-        plan.exec_ptr(a, b, method, st)
-    e1.record(torch.cuda.current_stream())
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / steps
+    res = []
+    for _ in range(blocks):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(torch.cuda.current_stream())
+        for i in range(steps):
+            a, b = bufs.pair(i)
+            plan.exec_ptr(a, b, method, st)
+        e1.record(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        res.append(e0.elapsed_time(e1) / steps)
+    return float(np.median(res))
 
 
 def per_kernel_ms(torch, plan, bufs, steps, method="forward"):
@@ -122,7 +149,7 @@ def traffic_for(key):
     return None
 
 
-def c2c_case(torch, n, batch, f64, steps, warmup, seed, check=8, uniform=False):
+def c2c_case(torch, n, batch, f64, steps, warmup, seed, check=8, uniform=False, blocks=1):
     """Throughput, per-kernel times and rel-L2 (first `check` transforms vs
     the FP64 oracle) of CFft1D forward on [batch, n]."""
     import oracle
@@ -138,7 +165,7 @@ def c2c_case(torch, n, batch, f64, steps, warmup, seed, check=8, uniform=False):
         else:
             a.normal_(generator=g)
     bufs = Bufs(torch, ib, ob, fill)
-    ms = time_plan(torch, p, bufs, steps, warmup)
+    ms = time_plan(torch, p, bufs, steps, warmup, blocks=blocks)
     kms = per_kernel_ms(torch, p, bufs, max(3, steps // 2))
     cdt = np.complex128 if f64 else np.complex64
     k = min(check, batch)
@@ -224,8 +251,8 @@ def suite(torch, args):
     # 1: plain method, CFft1D<f64> N=4096 forward then backward, batch 256 (+ FP32 form)
     recs = []
     for f64 in (True, False):
-        r, p, b, _ = c2c_case(torch, 4096, 256, f64, args.steps, args.warmup, 0x5EED0001, uniform=True)
-        msb = time_plan(torch, p, b, args.steps, args.warmup, "backward")
+        r, p, b, _ = c2c_case(torch, 4096, 256, f64, args.steps, args.warmup, 0x5EED0001, uniform=True, blocks=5)
+        msb = time_plan(torch, p, b, args.steps, args.warmup, "backward", blocks=5)
         r["ms_backward"] = round(msb, 5)
         recs.append(r)
     line("c1: CFft1D N=4096 batch 256, forward then backward", recs)
@@ -233,7 +260,8 @@ def suite(torch, args):
     recs = []
     for n, batch in ((4093, 2048), (10007, 512), (65537, 64)):
         for f64 in (False, True):
-            recs.append(c2c_case(torch, n, batch, f64, args.steps, args.warmup, 0x5EED0003, uniform=True)[0])
+            recs.append(c2c_case(torch, n, batch, f64, args.steps, args.warmup, 0x5EED0003, uniform=True,
+                                 blocks=5)[0])
     line("c3: Bluestein chirp-z, N in {4093, 10007, 65537}", recs)
     # 4: RFft1D 8192 x 4096, Dct1D II/III 1024 x 16384, Mdct1D 2048 hop 1024 over 2^24 samples
     recs = []
@@ -247,7 +275,7 @@ def suite(torch, args):
                 alg = 8 * n * batch * (2 if f64 else 1)
             ib, ob = p.io_bytes("forward")
             b = Bufs(torch, ib, ob, cplx_fill(torch, 0x5EED0004))
-            ms = time_plan(torch, p, b, args.steps, args.warmup)
+            ms = time_plan(torch, p, b, args.steps, args.warmup, blocks=5)
             # accuracy on 8 rows
             rows = 8
             xr = b.ins[0][:rows * n * (2 if f64 else 1)].view(torch.float64 if f64 else torch.float32)
@@ -284,7 +312,7 @@ def suite(torch, args):
 
             def pair(self, i):
                 return xs.data_ptr(), ys.data_ptr()
-        ms = time_plan(torch, p, One(), args.steps, args.warmup)
+        ms = time_plan(torch, p, One(), args.steps, args.warmup, blocks=5)
         p.exec_ptr(xs.data_ptr(), ys.data_ptr(), "forward", st)
         torch.cuda.synchronize()
         chk = 64
@@ -349,7 +377,7 @@ def suite(torch, args):
         def pair(self, i):
             return x2.data_ptr(), y2.data_ptr()
     ms_multi = time_plan(torch, fft.plan, P1(), max(3, args.steps // 4), 1)
-    ms_2d = time_plan(torch, p2, P2(), args.steps, args.warmup)
+    ms_2d = time_plan(torch, p2, P2(), args.steps, args.warmup, blocks=5)
     # rel-L2 per octave bucket on a sample of transforms (numpy FP64 reference)
     xs = x.cpu().numpy()
     ys = y.cpu().numpy()
@@ -391,6 +419,9 @@ def main():
     import torch
     import chfft_amd  # noqa: F401  (loads the HIP engine after torch's runtime)
     torch.cuda.init()
+    if args.copy_bytes:
+        print(json.dumps(copy_calibration(torch, args.copy_bytes)), flush=True)
+        return
     print(json.dumps(headline(torch, args)), flush=True)
     if args.suite:
         suite(torch, args)
