@@ -107,12 +107,23 @@ struct KParams {
 // 1: four-step (descs[desc], descs[desc+1]); 2: four-step Bluestein (three
 // descriptors); 3: length 1 (copy).
 struct MultiItem {
-  long long off;  // element offset of the transform in the in/out buffers
+  long long off;   // element offset of the transform in the in/out buffers
+  long long soff;  // element offset of its four-step intermediate in the scratch buffer
   int n;
   int kind;
   int desc;
   int pad;
 };
+
+// A unit of work of the multi-length launch: a whole fused item, or tiles
+// [g0, g1) of phase ph of a four-step item. Units are grouped by the launch's
+// phases; a grid-wide barrier separates the phases.
+struct MultiUnit {
+  int item;
+  int ph;
+  int g0, g1;
+};
+constexpr int kMultiPhases = 3;
 
 // Device table generation job (chfft_tables.hip).
 struct RootJob {

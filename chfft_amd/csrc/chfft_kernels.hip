@@ -198,50 +198,117 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? kMaxThreadsFsF64 : kMaxThread
   }
 }
 
-// Uneven multi-length batch in one persistent launch: workgroup w takes items
-// w, w + grid, w + 2*grid, ... (the planner sorts them by decreasing cost) and
-// runs each whole: in LDS when it fits, else as a four-step [Bluestein]
-// through the workgroup's scratch slot. Every loop bound is the item count,
-// so every wave exits.
+// Grid-wide barrier of the multi-length launch. Every workgroup of the grid is
+// resident at once (the planner sizes the grid to the occupancy), so a
+// workgroup that arrives waits only for the others to finish their share of
+// the phase. The wait is bounded: after kBarrierTicks of the 100 MHz
+// real-time counter (4 s) the workgroup records a failure in sync[1] and
+// leaves the kernel, so no wave can wait forever. The agent-scope fences
+// publish each workgroup's scratch writes to every XCD before it arrives, and
+// make the others' writes visible after.
+constexpr unsigned long long kBarrierTicks = 400000000ull;
+
+__device__ __forceinline__ bool grid_barrier(unsigned* sync, unsigned target) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    __hip_atomic_fetch_add(&sync[0], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    int good = 1;
+    const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+    while (__hip_atomic_load(&sync[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(8);
+      if (__builtin_amdgcn_s_memrealtime() - t0 > kBarrierTicks) {
+        good = 0;
+        __hip_atomic_fetch_or(&sync[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    ok = good;
+  }
+  __syncthreads();
+  __threadfence();
+  return ok != 0;
+}
+
+// Clears the barrier counter and the failure flag before each multi-length
+// launch (a kernel rather than a memset, so that the pair can be captured in
+// a graph and replayed).
+__global__ void multi_sync_reset(unsigned* sync) {
+  if (threadIdx.x < 2) sync[threadIdx.x] = 0u;
+}
+
+// Uneven multi-length batch in one persistent launch, in up to three phases
+// separated by grid-wide barriers: phase 0 runs the column passes of the
+// four-step items (chirp + columns for Bluestein), phase 1 their row passes
+// (spectrum product and inverse rows for Bluestein), phase 2 the inverse
+// columns + de-chirp of the Bluestein items; whole fused items (in LDS) fill
+// whichever phase the planner balanced them into. Within a phase, workgroup w
+// takes units w, w + grid, ... (the planner sorts each phase by decreasing
+// cost), so a long transform is spread over many workgroups instead of
+// bounding the launch. Every loop bound is a unit count and every barrier
+// wait is bounded, so every wave exits.
 template <typename T>
 __global__ void __launch_bounds__(sizeof(T) == 8 ? kMultiThreadsF64 : kMultiThreadsF32)
-    multi_kernel(const MultiItem* __restrict__ items, int nitems, const KParams* __restrict__ descs, const void* in,
-                 void* out, int scale_kind, void* scratch, long long slot_elems) {
+    multi_kernel(const MultiItem* __restrict__ items, const MultiUnit* __restrict__ units, int end0, int end1,
+                 int end2, const KParams* __restrict__ descs, const void* in, void* out, int scale_kind,
+                 void* scratch, unsigned* sync) {
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
-  cx<T>* slot = reinterpret_cast<cx<T>*>(scratch) + (long long)blockIdx.x * slot_elems;
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const MultiItem m = items[it];
-    const cx<T>* src = reinterpret_cast<const cx<T>*>(in) + m.off;
-    cx<T>* dst = reinterpret_cast<cx<T>*>(out) + m.off;
-    const double n = (double)m.n;
-    const double scale = scale_kind == 0 ? 1.0 : scale_kind == 1 ? 1.0 / sqrt(n) : 1.0 / n;
-    if (m.kind == 0) {
-      block_transform<T, LD_PLAIN, ST_PLAIN>(descs[m.desc], src, dst, lds, 0, 1, scale);
-      __syncthreads();
-    } else if (m.kind == 1) {
-      run_phase<T, LD_PLAIN, ST_TW>(descs[m.desc], src, slot, lds, 1.0);
-      run_phase<T, LD_PLAIN, ST_PLAIN>(descs[m.desc + 1], slot, dst, lds, scale);
-    } else if (m.kind == 2) {
-      run_phase<T, LD_FS_CHIRP, ST_TW>(descs[m.desc], src, slot, lds, 1.0);
-      run_phase<T, LD_PLAIN, ST_TW>(descs[m.desc + 1], slot, slot, lds, 1.0);
-      run_phase<T, LD_PLAIN, ST_FS_DECHIRP>(descs[m.desc + 2], slot, dst, lds, scale);
-    } else {
-      // length 1: returned unchanged (cfft1d.rs:213-214)
-      if (threadIdx.x == 0) dst[0] = src[0];
-      __syncthreads();
+  const int ends[kMultiPhases] = {end0, end1, end2};
+  const int nph = end2 > end1 ? 3 : end1 > end0 ? 2 : 1;
+  int u0 = 0;
+  for (int ph = 0; ph < nph; ph++) {
+    for (int u = u0 + blockIdx.x; u < ends[ph]; u += gridDim.x) {
+      const MultiUnit U = units[u];
+      const MultiItem m = items[U.item];
+      const cx<T>* src = reinterpret_cast<const cx<T>*>(in) + m.off;
+      cx<T>* dst = reinterpret_cast<cx<T>*>(out) + m.off;
+      cx<T>* slot = reinterpret_cast<cx<T>*>(scratch) + m.soff;
+      const double n = (double)m.n;
+      const double scale = scale_kind == 0 ? 1.0 : scale_kind == 1 ? 1.0 / sqrt(n) : 1.0 / n;
+      if (m.kind == 0) {
+        block_transform<T, LD_PLAIN, ST_PLAIN>(descs[m.desc], src, dst, lds, 0, 1, scale);
+        __syncthreads();
+      } else if (m.kind == 3) {
+        // length 1: returned unchanged (cfft1d.rs:213-214)
+        if (threadIdx.x == 0) dst[0] = src[0];
+      } else {
+        const KParams& Q = descs[m.desc + U.ph];
+        for (int g = U.g0; g < U.g1; g += Q.tpb) {
+          const int nt = min(Q.tpb, U.g1 - g);
+          if (m.kind == 1) {
+            if (U.ph == 0)
+              block_transform<T, LD_PLAIN, ST_TW>(Q, src, slot, lds, g, nt, 1.0);
+            else
+              block_transform<T, LD_PLAIN, ST_PLAIN>(Q, slot, dst, lds, g, nt, scale);
+          } else {
+            if (U.ph == 0)
+              block_transform<T, LD_FS_CHIRP, ST_TW>(Q, src, slot, lds, g, nt, 1.0);
+            else if (U.ph == 1)
+              block_transform<T, LD_PLAIN, ST_TW>(Q, slot, slot, lds, g, nt, 1.0);
+            else
+              block_transform<T, LD_PLAIN, ST_FS_DECHIRP>(Q, slot, dst, lds, g, nt, scale);
+          }
+          __syncthreads();  // LDS reuse by the next tile
+        }
+      }
     }
+    u0 = ends[ph];
+    if (ph + 1 < nph && !grid_barrier(sync, (unsigned)(ph + 1) * gridDim.x)) return;
   }
 }
 
-hipError_t launch_multi(bool f64, const MultiItem* items, int nitems, const KParams* descs, const void* in, void* out,
-                        int scale_kind, void* scratch, long long slot_elems, dim3 grid, size_t lds, hipStream_t st) {
+hipError_t launch_multi(bool f64, const MultiItem* items, const MultiUnit* units, const int* ends,
+                        const KParams* descs, const void* in, void* out, int scale_kind, void* scratch,
+                        unsigned* sync, dim3 grid, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL(multi_sync_reset, dim3(1), dim3(64), 0, st, sync);
   if (f64)
-    hipLaunchKernelGGL(multi_kernel<double>, grid, dim3(kMultiThreadsF64), lds, st, items, nitems, descs, in, out,
-                       scale_kind, scratch, slot_elems);
+    hipLaunchKernelGGL(multi_kernel<double>, grid, dim3(kMultiThreadsF64), lds, st, items, units, ends[0], ends[1],
+                       ends[2], descs, in, out, scale_kind, scratch, sync);
   else
-    hipLaunchKernelGGL(multi_kernel<float>, grid, dim3(kMultiThreadsF32), lds, st, items, nitems, descs, in, out,
-                       scale_kind, scratch, slot_elems);
+    hipLaunchKernelGGL(multi_kernel<float>, grid, dim3(kMultiThreadsF32), lds, st, items, units, ends[0], ends[1],
+                       ends[2], descs, in, out, scale_kind, scratch, sync);
   return hipGetLastError();
 }
 
@@ -317,8 +384,8 @@ hipError_t set_fused_lds_limits() {
                        reinterpret_cast<const void*>(&fourstep_kernel<double, 3>)};
   const void* ms[2] = {reinterpret_cast<const void*>(&multi_kernel<float>),
                        reinterpret_cast<const void*>(&multi_kernel<double>)};
-  for (const void* f : ms) {
-    hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes);
+  for (const void* f : ms) {  // these keep a few bytes of static LDS (the barrier flag)
+    hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes - 64);
     if (r != hipSuccess) e = r;
   }
   for (const void* f : fs) {

@@ -41,8 +41,9 @@ hipError_t launch_fourstep(bool f64, int nph, const KParams* Q, const void* in, 
                            long long src_dist, long long dst_dist, void* scratch, long long slot_elems, dim3 grid,
                            dim3 block, size_t lds, hipStream_t stream);
 int fourstep_occupancy(bool f64, int nph, int block, size_t lds);
-hipError_t launch_multi(bool f64, const MultiItem* items, int nitems, const KParams* descs, const void* in, void* out,
-                        int scale_kind, void* scratch, long long slot_elems, dim3 grid, size_t lds, hipStream_t st);
+hipError_t launch_multi(bool f64, const MultiItem* items, const MultiUnit* units, const int* ends,
+                        const KParams* descs, const void* in, void* out, int scale_kind, void* scratch,
+                        unsigned* sync, dim3 grid, size_t lds, hipStream_t st);
 int multi_occupancy(bool f64, size_t lds);
 hipError_t gen_roots(bool f64, const RootJob* jobs_dev, int njobs, void* arena, hipStream_t st);
 hipError_t gen_rot(const long long* ns_dev, int count, long long M, void* out, hipStream_t st);
@@ -933,8 +934,8 @@ struct chfft_plan_s {
   std::vector<size_t> lens;
   size_t total = 0;  // complex elements over all transforms
   int nitems = 0;
-  DevBuf arena, items_dev, descs_fwd, descs_bwd;
-  long long slot_elems = 0;
+  DevBuf arena, items_dev, units_dev, descs_fwd, descs_bwd, sync_dev;
+  int unit_ends[kMultiPhases] = {0, 0, 0};  // units of phases 0..2 end here
   size_t multi_lds = 0;
   int multi_grid = 0, multi_occ = 1;
   std::string multi_desc;
@@ -1372,9 +1373,10 @@ int chfft_exec_range(chfft_plan plan, const void* in, void* out, int method, int
     if (first != 0 || count != 1) return count == 0 ? CHFFT_OK : CHFFT_ERR_INVALID_ARGUMENT;
     if (plan->nitems == 0) return CHFFT_OK;
     const KParams* descs = reinterpret_cast<const KParams*>(inverse ? plan->descs_bwd.p : plan->descs_fwd.p);
-    if (launch_multi(plan->f64, reinterpret_cast<const MultiItem*>(plan->items_dev.p), plan->nitems, descs, in, out,
-                     (int)scale, plan->scratch.p, plan->slot_elems, dim3((unsigned)plan->multi_grid),
-                     plan->multi_lds, st) != hipSuccess)
+    if (launch_multi(plan->f64, reinterpret_cast<const MultiItem*>(plan->items_dev.p),
+                     reinterpret_cast<const MultiUnit*>(plan->units_dev.p), plan->unit_ends, descs, in, out,
+                     (int)scale, plan->scratch.p, reinterpret_cast<unsigned*>(plan->sync_dev.p),
+                     dim3((unsigned)plan->multi_grid), plan->multi_lds, st) != hipSuccess)
       return CHFFT_ERR_HIP;
     return CHFFT_OK;
   }
@@ -1461,6 +1463,12 @@ int chfft_exec_host(chfft_plan plan, const void* in, void* out, int method) {
   if (hipMemcpy(plan->stage_in.p, in, ib, hipMemcpyHostToDevice) != hipSuccess) return CHFFT_ERR_HIP;
   if ((rc = chfft_exec(plan, plan->stage_in.p, plan->stage_out.p, method, nullptr))) return rc;
   if (hipMemcpy(out, plan->stage_out.p, ob, hipMemcpyDeviceToHost) != hipSuccess) return CHFFT_ERR_HIP;
+  if (plan->kind == K_MULTI) {
+    // a grid-wide barrier that timed out leaves its flag behind
+    unsigned sync[2] = {0, 0};
+    if (hipMemcpy(sync, plan->sync_dev.p, sizeof sync, hipMemcpyDeviceToHost) != hipSuccess) return CHFFT_ERR_HIP;
+    if (sync[1]) return CHFFT_ERR_HIP;
+  }
   return CHFFT_OK;
 }
 
@@ -1525,7 +1533,8 @@ int phase_tpb(size_t L, int block, bool colmajor, bool f64) {
     const int tpt = block / tpb;
     size_t Lp = L + (L + 15) / 16;
     if (tpb > 1 && (Lp & 1) == 0) Lp++;
-    if ((size_t)tpt * kEmax >= L && (size_t)tpb * Lp * esz <= kLdsBytes) break;
+    // (the multi-length kernel also keeps a few bytes of static LDS)
+    if ((size_t)tpt * kEmax >= L && (size_t)tpb * Lp * esz + 64 <= kLdsBytes) break;
     tpb >>= 1;
   }
   return tpb;
@@ -1713,7 +1722,6 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
   auto at = [&](long long off) { return (const void*)(base + (size_t)off * esz); };
   std::vector<KParams> dfw, dbw;
   size_t lds = 0;
-  long long slot = 0;
   for (auto& kv : info) {
     LenInfo& L = kv.second;
     if (L.kind == 3) continue;
@@ -1793,48 +1801,93 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
         D.push_back(A);
         D.push_back(B);
         if (L.kind == 2) D.push_back(C);
-        slot = std::max(slot, (long long)L.M);
       }
     }
     L.ndesc = (int)dfw.size() - L.desc;
   }
-  // items: offsets in the packed buffers, largest work first
+  // items: offsets in the packed buffers and, for four-step items, in the
+  // scratch buffer (each keeps its intermediate from one phase to the next)
   std::vector<MultiItem> items(count);
-  std::vector<size_t> off(count);
   size_t acc = 0;
+  long long sacc = 0;
   for (size_t i = 0; i < count; i++) {
-    off[i] = acc;
+    const LenInfo& L = info[lens[i]];
+    items[i].off = (long long)acc;
     acc += lens[i];
+    items[i].n = (int)lens[i];
+    items[i].kind = L.kind;
+    items[i].desc = L.desc;
+    items[i].soff = 0;
+    if (L.kind == 1 || L.kind == 2) {
+      items[i].soff = sacc;
+      sacc += (long long)L.M;
+    }
   }
-  std::vector<size_t> order(count);
-  for (size_t i = 0; i < count; i++) order[i] = i;
-  auto cost = [&](size_t i) {
+  // units: tiles of each four-step phase in chunks of about kChunk points,
+  // whole fused items; fused items go to the phase with the least work so
+  // far; each phase sorted by decreasing cost (workgroup w takes units w,
+  // w + grid, ...)
+  const double kChunk = 32768.0;
+  std::vector<std::pair<double, MultiUnit>> ph_units[kMultiPhases];
+  double ph_cost[kMultiPhases] = {0, 0, 0};
+  int nph = 1;
+  for (size_t i = 0; i < count; i++) {
     const LenInfo& L = info[lens[i]];
+    if (L.kind == 1 || L.kind == 2) {
+      const int np = L.kind == 1 ? 2 : 3;
+      nph = std::max(nph, np);
+      for (int ph = 0; ph < np; ph++) {
+        const KParams& Q = dfw[L.desc + ph];
+        const double tile_pts = (double)Q.tpb * Q.Lfft;
+        const int tiles = (Q.G + Q.tpb - 1) / Q.tpb;
+        const int per = std::max(1, (int)(kChunk / tile_pts));
+        for (int t0 = 0; t0 < tiles; t0 += per) {
+          MultiUnit U{(int)i, ph, t0 * Q.tpb, std::min(Q.G, (t0 + per) * Q.tpb)};
+          const double c = (double)(U.g1 - U.g0) * Q.Lfft * std::log2((double)Q.Lfft + 1.0) * (ph == 1 && L.kind == 2 ? 2.0 : 1.0);
+          ph_units[ph].push_back({c, U});
+          ph_cost[ph] += c;
+        }
+      }
+    }
+  }
+  for (size_t i = 0; i < count; i++) {
+    const LenInfo& L = info[lens[i]];
+    if (L.kind == 1 || L.kind == 2) continue;
     const double m = (double)std::max<size_t>(L.M, 1);
-    return m * std::log2(m + 1) * (L.kind == 2 ? 3.0 : L.kind == 1 ? 2.0 : L.blue ? 2.0 : 1.0);
-  };
-  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
-  for (size_t k = 0; k < count; k++) {
-    const size_t i = order[k];
-    const LenInfo& L = info[lens[i]];
-    items[k].off = (long long)off[i];
-    items[k].n = (int)lens[i];
-    items[k].kind = L.kind;
-    items[k].desc = L.desc;
+    const double c = L.kind == 3 ? 1.0 : m * std::log2(m + 1.0) * (L.blue ? 2.0 : 1.0);
+    int best = 0;
+    for (int ph = 1; ph < nph; ph++)
+      if (ph_cost[ph] < ph_cost[best]) best = ph;
+    ph_units[best].push_back({c, MultiUnit{(int)i, 0, 0, 1}});
+    ph_cost[best] += c;
+  }
+  std::vector<MultiUnit> units;
+  for (int ph = 0; ph < kMultiPhases; ph++) {
+    std::stable_sort(ph_units[ph].begin(), ph_units[ph].end(),
+                     [](const std::pair<double, MultiUnit>& a, const std::pair<double, MultiUnit>& b) {
+                       return a.first > b.first;
+                     });
+    for (auto& cu : ph_units[ph]) units.push_back(cu.second);
+    p->unit_ends[ph] = (int)units.size();
   }
   if ((rc = dev_alloc(p->items_dev, std::max<size_t>(count, 1) * sizeof(MultiItem)))) return rc;
+  if ((rc = dev_alloc(p->units_dev, std::max<size_t>(units.size(), 1) * sizeof(MultiUnit)))) return rc;
   if ((rc = dev_alloc(p->descs_fwd, std::max<size_t>(dfw.size(), 1) * sizeof(KParams)))) return rc;
   if ((rc = dev_alloc(p->descs_bwd, std::max<size_t>(dbw.size(), 1) * sizeof(KParams)))) return rc;
+  if ((rc = dev_alloc(p->sync_dev, 2 * sizeof(unsigned)))) return rc;
   if ((count && hipMemcpy(p->items_dev.p, items.data(), count * sizeof(MultiItem), hipMemcpyHostToDevice)) ||
+      (!units.empty() &&
+       hipMemcpy(p->units_dev.p, units.data(), units.size() * sizeof(MultiUnit), hipMemcpyHostToDevice)) ||
       (!dfw.empty() && hipMemcpy(p->descs_fwd.p, dfw.data(), dfw.size() * sizeof(KParams), hipMemcpyHostToDevice)) ||
-      (!dbw.empty() && hipMemcpy(p->descs_bwd.p, dbw.data(), dbw.size() * sizeof(KParams), hipMemcpyHostToDevice)))
+      (!dbw.empty() && hipMemcpy(p->descs_bwd.p, dbw.data(), dbw.size() * sizeof(KParams), hipMemcpyHostToDevice)) ||
+      hipMemset(p->sync_dev.p, 0, 2 * sizeof(unsigned)) != hipSuccess)
     return CHFFT_ERR_HIP;
   p->nitems = (int)count;
   p->multi_lds = std::max<size_t>(lds, 16);
   p->multi_occ = multi_occupancy(f64, p->multi_lds);
-  p->multi_grid = std::max(1, std::min((int)count, p->multi_occ * C2CEngine::num_cus()));
-  p->slot_elems = slot;
-  if ((rc = dev_alloc(p->scratch, (size_t)p->multi_grid * (size_t)slot * esz))) return rc;
+  // every workgroup resident at once (the phases meet at grid-wide barriers)
+  p->multi_grid = std::max(1, p->multi_occ * C2CEngine::num_cus());
+  if ((rc = dev_alloc(p->scratch, (size_t)std::max<long long>(sacc, 1) * esz))) return rc;
   size_t nf = 0, nf4 = 0, nb4 = 0;
   for (auto& kv : info) {
     nf += kv.second.kind == 0;
@@ -1844,24 +1897,23 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
   p->multi_desc = "multi-length batch: " + std::to_string(count) + " transforms, " + std::to_string(total) +
                   " points, " + std::to_string(info.size()) + " distinct lengths (" + std::to_string(nf) +
                   " fused in LDS, " + std::to_string(nf4) + " four-step, " + std::to_string(nb4) +
-                  " four-step Bluestein) | one persistent launch: grid=" + std::to_string(p->multi_grid) +
+                  " four-step Bluestein) | one persistent launch, " + std::to_string(nph) + " phase(s), " +
+                  std::to_string(units.size()) + " work units: grid=" + std::to_string(p->multi_grid) +
                   " block=" + std::to_string(block) + " lds=" + std::to_string(p->multi_lds) +
-                  " scratch=" + std::to_string((size_t)p->multi_grid * (size_t)slot * esz) + "\n";
+                  " scratch=" + std::to_string((size_t)std::max<long long>(sacc, 1) * esz) + "\n";
   *plan = p.release();
   return CHFFT_OK;
 }
 
 // Sets the number of workgroups of a multi-length plan's persistent launch
-// (the autotuner's knob); 0 restores the occupancy-based default.
+// (the autotuner's knob); 0 restores the default, every workgroup that fits
+// at once. Larger grids are refused: the launch's phases meet at grid-wide
+// barriers, which need every workgroup resident.
 extern "C" int chfft_plan_set_grid(chfft_plan plan, int grid) {
   if (!plan || plan->kind != K_MULTI || grid < 0) return CHFFT_ERR_INVALID_ARGUMENT;
-  const int def = std::max(1, std::min(plan->nitems, plan->multi_occ * C2CEngine::num_cus()));
-  const int g = grid ? grid : def;
-  if ((size_t)g * (size_t)plan->slot_elems * esz_of(plan->f64) > plan->scratch.bytes) {
-    int rc = dev_alloc(plan->scratch, (size_t)g * (size_t)plan->slot_elems * esz_of(plan->f64));
-    if (rc) return rc;
-  }
-  plan->multi_grid = g;
+  const int cap = std::max(1, plan->multi_occ * C2CEngine::num_cus());
+  if (grid > cap) return CHFFT_ERR_INVALID_ARGUMENT;
+  plan->multi_grid = grid ? grid : cap;
   return CHFFT_OK;
 }
 
@@ -1922,10 +1974,10 @@ extern "C" int chfft_plan_autotune(chfft_plan plan, void* in, void* out, int rep
   int best = -1;
   double best_ms = 1e30;
   if (plan->kind == K_MULTI) {
-    const int def = std::max(1, std::min(plan->nitems, plan->multi_occ * C2CEngine::num_cus()));
-    const int cand[5] = {def / 4, def / 2, def, def * 2, def * 4};
-    for (int i = 0; i < 5; i++) {
-      const int g = std::max(1, std::min(cand[i], std::max(1, plan->nitems)));
+    const int def = std::max(1, plan->multi_occ * C2CEngine::num_cus());
+    const int cand[3] = {std::max(1, def / 4), std::max(1, def / 2), def};
+    for (int i = 0; i < 3; i++) {
+      const int g = cand[i];
       if (chfft_plan_set_grid(plan, g) != CHFFT_OK) continue;
       const double ms = time_exec(plan, in, out, reps, st);
       report += " grid=" + std::to_string(g) + ":" + std::to_string(ms) + "ms";
@@ -1935,7 +1987,7 @@ extern "C" int chfft_plan_autotune(chfft_plan plan, void* in, void* out, int rep
       }
     }
     if (best < 0) return -CHFFT_ERR_HIP;
-    chfft_plan_set_grid(plan, std::max(1, std::min(cand[best], std::max(1, plan->nitems))));
+    chfft_plan_set_grid(plan, cand[best]);
     plan->multi_desc += report + " -> grid " + std::to_string(plan->multi_grid) + "\n";
     return best;
   }
