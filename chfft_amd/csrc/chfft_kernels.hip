@@ -93,16 +93,31 @@ __device__ void load_stage(const KParams& P, const TMap& m, const void* src, cx<
   cx<T>* row = lds + t * P.Lp;
   if (t < nt) {
     const Loader<T, LD> ld = make_loader<T, LD>(P, src, g0 + t);
-    for (int n = u; n < P.Lfft; n += m.tpt) {
-      cx<T> v = mk(T(0), T(0));
-      if (n < P.L) {  // n >= L only under a fused Bluestein: zero padding
-        v = load_elem<T, LD>(P, ld, n);
-        if constexpr (LD != LD_FS_CHIRP) {
-          if (P.conj_in) v = cj(v);
-        }
-        if (P.has_chirp) v = v * chirp[n];
+    // batches of 8 elements per thread: all loads of a batch are issued
+    // before any LDS write, so they are in flight together
+    constexpr int NB = 8;
+    for (int n0 = u; n0 < P.Lfft; n0 += NB * m.tpt) {
+      cx<T> v[NB];
+#pragma unroll
+      for (int e = 0; e < NB; e++) {
+        const int n = n0 + e * m.tpt;
+        v[e] = mk(T(0), T(0));
+        if (n < P.L) v[e] = load_elem<T, LD>(P, ld, n);  // n >= L only under a fused Bluestein: zero padding
       }
-      row[lx(n)] = v;
+#pragma unroll
+      for (int e = 0; e < NB; e++) {
+        const int n = n0 + e * m.tpt;
+        if (n < P.Lfft) {
+          cx<T> x = v[e];
+          if (n < P.L) {
+            if constexpr (LD != LD_FS_CHIRP) {
+              if (P.conj_in) x = cj(x);
+            }
+            if (P.has_chirp) x = x * chirp[n];
+          }
+          row[lx(n)] = x;
+        }
+      }
     }
   } else {
     for (int n = u; n < P.Lfft; n += m.tpt) row[lx(n)] = mk(T(0), T(0));
@@ -250,6 +265,7 @@ __global__ void multi_sync_reset(unsigned* sync) {
 // wait is bounded, so every wave exits.
 template <typename T>
 __global__ void __launch_bounds__(sizeof(T) == 8 ? kMultiThreadsF64 : kMultiThreadsF32)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 4)))
     multi_kernel(const MultiItem* __restrict__ items, const MultiUnit* __restrict__ units, int end0, int end1,
                  int end2, const KParams* __restrict__ descs, const void* in, void* out, int scale_kind,
                  void* scratch, unsigned* sync) {

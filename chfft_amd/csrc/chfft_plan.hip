@@ -241,6 +241,13 @@ bool smooth7(size_t n) {
   return n == 1;
 }
 
+bool smooth5(size_t n) {
+  if (n == 0) return false;
+  for (size_t p : {2, 3, 5})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
 size_t next_pow2(size_t n) {
   size_t p = 1;
   while (p < n) p <<= 1;
@@ -447,10 +454,7 @@ size_t bluestein_length(size_t N, bool f64) {
   size_t best = hi;
   double best_cost = 1e300;
   for (size_t m = 2 * N - 1; m <= hi; m++) {
-    size_t r = m;
-    for (size_t q : {2, 3, 5})
-      while (r % q == 0) r /= q;
-    if (r != 1) continue;
+    if (!smooth5(m)) continue;
     const size_t d = fourstep_split(m, f64);
     if (d == 0 || (m / d) % 16 != 0 || (m / d) * 4 > lds_cap(f64)) continue;
     const double cost = (double)m * (double)(radices_for(d).size() + radices_for(m / d).size());
@@ -1614,7 +1618,20 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
       L.kind = 3;
     } else {
       const bool sm = smooth7(n);
-      const size_t M = sm ? n : 2 * next_pow2(n);
+      // Bluestein padding: the smallest 5-smooth M >= 2N - 1 that one
+      // workgroup holds or that splits into two such factors (every such M
+      // gives the reference's linear convolution, cfft1d.rs:114 pads to
+      // 2 * next_pow2(N))
+      size_t M = sm ? n : 2 * next_pow2(n);
+      if (!sm)
+        for (size_t m = 2 * n - 1; m < M; m++) {
+          if (!smooth5(m)) continue;
+          const size_t d = m <= capM ? 1 : fourstep_split(m, f64);
+          if (d && m / d <= capM) {
+            M = m;
+            break;
+          }
+        }
       L.blue = !sm;
       L.M = M;
       if (M <= capM) {
