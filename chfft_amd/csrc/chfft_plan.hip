@@ -1618,14 +1618,17 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
       L.kind = 3;
     } else {
       const bool sm = smooth7(n);
-      // Bluestein padding: the smallest 5-smooth M >= 2N - 1 that one
-      // workgroup holds or that splits into two such factors (every such M
-      // gives the reference's linear convolution, cfft1d.rs:114 pads to
-      // 2 * next_pow2(N))
+      // Bluestein padding: the smallest M >= 2N - 1 of the form 2^a times
+      // 1, 3, 5 or 15 (at most 25 % above the bound, and few distinct M per
+      // batch, each of which needs its own spectrum plan) that one workgroup
+      // holds or that splits into two such factors; every such M gives the
+      // reference's linear convolution (cfft1d.rs:114 pads to 2 * next_pow2(N))
       size_t M = sm ? n : 2 * next_pow2(n);
       if (!sm)
         for (size_t m = 2 * n - 1; m < M; m++) {
-          if (!smooth5(m)) continue;
+          size_t odd = m;
+          while (odd % 2 == 0) odd /= 2;
+          if (odd != 1 && odd != 3 && odd != 5 && odd != 15) continue;
           const size_t d = m <= capM ? 1 : fourstep_split(m, f64);
           if (d && m / d <= capM) {
             M = m;
