@@ -384,24 +384,25 @@ __device__ void store_stage(const KParams& P, const TMap& m, void* dst, const cx
   };
   const long long obase = (long long)(g0 + t) * P.out.bdist;
   if constexpr (ST == ST_R2C) {
-    // rfft1d.rs:134-148: N/2+1 bins per row
+    // rfft1d.rs:134-148: N/2+1 bins per row. Bins i and h - i (0 < i < q)
+    // come from the same pair Z(i), Z(h - i) and twiddle co[i]: one thread
+    // computes both; i = 0 gives bins 0 and h, i = q (N % 4 == 0) bin q.
     const int N = (int)P.N, h = N >> 1, q = (N + 3) >> 2;
     const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
     cx<T>* out = reinterpret_cast<cx<T>*>(dst) + obase;
-    for (int r = m.ur; r <= h; r += m.tpt) {
-      cx<T> v;
-      if (r == 0 || r == h) {
+    for (int i = m.ur; i <= q; i += m.tpt) {
+      if (i == 0) {
         const cx<T> z = Z(0);
-        v = mk(r == 0 ? z.re + z.im : z.re - z.im, T(0));
-      } else if ((N & 3) == 0 && r == q) {
-        v = cj(Z(q));
-      } else {
-        const int i = (r < q) ? r : h - r;
+        out[0] = scl(mk(z.re + z.im, T(0)), s);
+        out[h] = scl(mk(z.re - z.im, T(0)), s);
+      } else if (i < q) {
         const cx<T> zi = Z(i), zh = Z(h - i);
         const cx<T> x = co[i] * (zi - cj(zh));
-        v = (r < q) ? zi - x : zh + cj(x);
+        out[i] = scl(zi - x, s);
+        out[h - i] = scl(zh + cj(x), s);
+      } else if ((N & 3) == 0) {
+        out[q] = scl(cj(Z(q)), s);
       }
-      out[r] = scl(v, s);
     }
   } else if constexpr (ST == ST_DCT2) {
     // dct1d.rs:266-287

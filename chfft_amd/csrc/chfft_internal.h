@@ -23,6 +23,18 @@ constexpr int kMultiThreadsF64 = 256;
 // LDS per workgroup on gfx950.
 constexpr size_t kLdsBytes = 160 * 1024;
 
+// Compact twiddle tables of the length-specialised kernels. Every pass table
+// [j-1][k] = W_{pR}^(j k) (span p, radix R) that would exceed kTw2Min entries
+// is kept in a two-level form, A[j-1][k mod 16] = W_{pR}^(j (k mod 16)) and
+// B[j-1][k / 16] = W_{pR}^(16 j (k / 16)), whose product is the twiddle; the
+// other passes keep their full table. The compact tables of all passes follow
+// the full tables in the plan's twiddle buffer, pass after pass.
+constexpr int kTw2Min = 1024;
+constexpr bool tw_two_level(int R, int p) { return p % 16 == 0 && (R - 1) * p > kTw2Min; }
+constexpr int tw_compact_entries(int R, int p) {
+  return p <= 1 ? 0 : tw_two_level(R, p) ? (R - 1) * (16 + p / 16) : (R - 1) * p;
+}
+
 // One radix-R Stockham pass over a length-L sequence resident in LDS.
 // p = product of the radices already applied; tws = L / (p * R) is the step of
 // the twiddle index j*k*tws into the W_L table.

@@ -402,6 +402,27 @@ int build_seq(size_t L, bool f64, SeqDesc& seq, DevBuf& tw) {
     }
     p *= R;
   }
+  // compact tables (the length-specialised kernels' form), pass after pass
+  p = 1;
+  for (size_t q = 0; q < rad.size(); q++) {
+    const int R = rad[q];
+    auto put = [&](unsigned long long num, unsigned long long den) {
+      ld_t re, im;
+      unit_root(num, den, &re, &im);
+      tab.push_back(re);
+      tab.push_back(im);
+    };
+    if (p > 1 && tw_two_level(R, p)) {
+      for (int j = 1; j < R; j++)
+        for (int k = 0; k < 16; k++) put((unsigned long long)j * k, (unsigned long long)p * R);
+      for (int j = 1; j < R; j++)
+        for (int k = 0; k < p / 16; k++) put((unsigned long long)j * k * 16, (unsigned long long)p * R);
+    } else if (p > 1) {
+      for (int j = 1; j < R; j++)
+        for (int k = 0; k < p; k++) put((unsigned long long)j * k, (unsigned long long)p * R);
+    }
+    p *= R;
+  }
   if (tab.empty()) {
     tab.push_back(1.0L);
     tab.push_back(0.0L);

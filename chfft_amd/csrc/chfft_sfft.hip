@@ -57,6 +57,15 @@ constexpr int rad_twoff(Rad<RS...> r, int q) {
   return off;
 }
 
+// offset of pass q's compact table (tw_compact_entries) from the start of
+// the compact region, which follows the full tables
+template <int... RS>
+constexpr int rad_compact_off(Rad<RS...> r, int q) {
+  int off = 0;
+  for (int i = 0; i < q; i++) off += tw_compact_entries(rad_at(r, i), rad_span(r, i));
+  return off;
+}
+
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // Register budgets (bytes per thread) for twiddles fetched early: a pass's
@@ -68,13 +77,18 @@ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 #ifndef CHFFT_TW_EARLY_BYTES
 #define CHFFT_TW_EARLY_BYTES 96
 #endif
-// Per-pass twiddle tables up to this size are copied into LDS once per tile
-// and read from there (shorter latency than the L2, and off the vector
-// memory path).
-#ifndef CHFFT_TW_LDS_BYTES
-#define CHFFT_TW_LDS_BYTES 2048
+// Per-pass twiddle tables up to this size (per precision) are copied into
+// LDS once per tile and read from there (shorter latency than the L2, and
+// off the vector memory path).
+#ifndef CHFFT_TW_LDS_BYTES_F32
+#define CHFFT_TW_LDS_BYTES_F32 8192
 #endif
-constexpr bool tw_in_lds(int entries, int esz) { return entries > 0 && entries * esz <= CHFFT_TW_LDS_BYTES; }
+#ifndef CHFFT_TW_LDS_BYTES_F64
+#define CHFFT_TW_LDS_BYTES_F64 2048
+#endif
+constexpr bool tw_in_lds(int entries, int esz) {
+  return entries > 0 && entries * esz <= (esz == 8 ? CHFFT_TW_LDS_BYTES_F32 : CHFFT_TW_LDS_BYTES_F64);
+}
 constexpr int lds_stride(int L, int tpb) {
   return (tpb > 1 && ((L + (L + 15) / 16) & 1) == 0) ? L + (L + 15) / 16 + 1 : L + (L + 15) / 16;
 }
@@ -162,17 +176,23 @@ struct TwReg {
   // small sets are fetched one pass early, before the previous pass computes
   static constexpr bool AHEAD = N * (int)sizeof(cx<T>) <= CHFFT_TW_AHEAD_BYTES;
   cx<T> w[N > 0 ? N : 1];
-  // LDT: tw is the table's copy in LDS
+  // LDT: tw is the LDS copy of the compact tables (two-level passes multiply
+  // their two factors); else the full tables in global memory
   template <bool LDT>
   __device__ __forceinline__ void load(int u, const cx<T>* __restrict__ tw) {
     constexpr int off = rad_twoff(RS{}, q);
+    constexpr int coff = rad_compact_off(RS{}, q);
+    constexpr bool TWO = tw_two_level(R, p);
 #pragma unroll
     for (int g = 0; g < NG; g++) {
       const unsigned k = (unsigned)(u + g * TPT) % (unsigned)p;
 #pragma unroll
       for (int j = 1; j < R; j++) {
-        if constexpr (LDT)
-          w[g * (R - 1) + j - 1] = tw[off + (j - 1) * p + k];
+        if constexpr (LDT && TWO)
+          w[g * (R - 1) + j - 1] = tw[coff + (j - 1) * 16 + (k & 15)] *
+                                   tw[coff + (R - 1) * 16 + (j - 1) * (p / 16) + (k >> 4)];
+        else if constexpr (LDT)
+          w[g * (R - 1) + j - 1] = tw[coff + (j - 1) * p + k];
         else
           w[g * (R - 1) + j - 1] = ld_off<cx<T>>(tw + off + (j - 1) * p, k * (unsigned)sizeof(cx<T>));
       }
@@ -463,9 +483,10 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
   // a second LDS buffer saves one barrier per exchange but halves the
   // workgroups that fit on a CU; only tiny tiles take it
   constexpr bool PP = 2ull * TPB * LP * sizeof(cx<T>) <= 16384;
-  constexpr int NTW = rad_twoff(RS{}, NP);
-  constexpr bool LDT = tw_in_lds(NTW, (int)sizeof(cx<T>));
-  static_assert(((PP ? 2 : 1) * TPB * LP + (LDT ? NTW : 0)) * sizeof(cx<T>) <= kLdsBytes,
+  constexpr int NTW = rad_twoff(RS{}, NP);          // full tables
+  constexpr int NTWC = rad_compact_off(RS{}, NP);   // compact tables, after the full ones
+  constexpr bool LDT = tw_in_lds(NTWC, (int)sizeof(cx<T>));
+  static_assert(((PP ? 2 : 1) * TPB * LP + (LDT ? NTWC : 0)) * sizeof(cx<T>) <= kLdsBytes,
                 "transform tile does not fit in LDS");
   const int tid = threadIdx.x;
   const int t = COL ? tid % TPB : tid / TPT;
@@ -490,8 +511,8 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
     cx<T>* twl = lds + (PP ? 2 : 1) * TPB * LP;
     constexpr int TH = TPT * TPB;
 #pragma unroll
-    for (int e0 = 0; e0 < NTW; e0 += TH)
-      if (e0 + tid < NTW) twl[e0 + tid] = ld_off<cx<T>>(tw + e0, (unsigned)tid * ESZ);
+    for (int e0 = 0; e0 < NTWC; e0 += TH)
+      if (e0 + tid < NTWC) twl[e0 + tid] = ld_off<cx<T>>(tw + NTW + e0, (unsigned)tid * ESZ);
     twp = twl;
   }
   constexpr bool TW_EARLY = StoreTw<T, PL, TPT, ST, EPI>::EARLY;
@@ -768,7 +789,7 @@ struct SfsEntry {
 
 template <int L, int TPB, bool F64, class RS>
 constexpr size_t tile_bytes() {
-  return lds_need(L, TPB, F64 ? 16 : 8, rad_twoff(RS{}, rad_count(RS{})));
+  return lds_need(L, TPB, F64 ? 16 : 8, rad_compact_off(RS{}, rad_count(RS{})));
 }
 constexpr size_t mx(size_t a, size_t b) { return a > b ? a : b; }
 
@@ -850,9 +871,9 @@ hipError_t sfft_set_lds_limits() {
 
 // LDS bytes an entry needs (two buffers when they fit, else one).
 size_t sfft_lds_bytes(const SfftEntry* e) {
-  int ntw = 0, p = 1;
+  int ntw = 0, p = 1;  // compact table entries
   for (int q = 0; q < e->nrad; q++) {
-    if (p > 1) ntw += (e->rad[q] - 1) * p;
+    ntw += tw_compact_entries(e->rad[q], p);
     p *= e->rad[q];
   }
   return lds_need(e->L, e->tpb, e->f64 ? 16 : 8, ntw);
