@@ -639,7 +639,17 @@ struct C2CEngine {
     if ((rc = upload(fourstep_table(n1, n2), f64, twN))) return rc;
     if (mode == 3) {
       if ((rc = upload(chirp_table(len), f64, chirp))) return rc;
-      if ((rc = upload(bluestein_spectrum(len, total), f64, kspec))) return rc;
+      // the row pass reads the spectrum at bins k1 + n1 * k2 (row k1, element
+      // k2): stored row-major, [k1][k2], so that consecutive lanes read
+      // consecutive entries
+      const std::vector<ld_t> K = bluestein_spectrum(len, total);
+      std::vector<ld_t> Kt(K.size());
+      for (size_t k1 = 0; k1 < n1; k1++)
+        for (size_t k2 = 0; k2 < n2; k2++) {
+          Kt[2 * (k1 * n2 + k2)] = K[2 * (k1 + n1 * k2)];
+          Kt[2 * (k1 * n2 + k2) + 1] = K[2 * (k1 + n1 * k2) + 1];
+        }
+      if ((rc = upload(Kt, f64, kspec))) return rc;
     }
     return CHFFT_OK;
   }
@@ -751,8 +761,8 @@ struct C2CEngine {
     b.P.out = io((long long)M, (long long)n2, 1, (int)n1);
     b.P.has_mid = 1;
     b.P.kspec = kspec.p;
-    b.P.kc = 1;
-    b.P.ke = (long long)n1;
+    b.P.kc = (long long)n2;
+    b.P.ke = 1;
     b.P.twN = (long long)M;
     b.P.tab_a = twN.p;
     b.P.twk = 1;
@@ -841,8 +851,8 @@ struct C2CEngine {
       B.out = B.in;
       B.has_mid = 1;
       B.kspec = kspec.p;
-      B.kc = 1;
-      B.ke = (long long)n1;
+      B.kc = (long long)n2;
+      B.ke = 1;
       B.twN = (long long)M;
       B.tab_a = twN.p;
       B.twk = 1;
