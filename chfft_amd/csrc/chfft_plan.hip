@@ -560,6 +560,10 @@ void try_specialise(Launch& l, bool f64) {
     const long long sp = tile_span_bytes(l.P.in, l.P.G, e->tpb, l.P.Lfft, esz);
     if (sp < 0 || sp >= lim) return;
   }
+  if (l.ld == LD_FS_CHIRP) {  // sequence offsets s < N, and every s the tile forms, in 32 bits
+    const long long smax = (long long)(l.P.in.ncols - 1) * l.P.in.cstride + (long long)(l.P.Lfft - 1) * l.P.in.estride;
+    if (l.P.N * (long long)esz >= lim || smax >= lim || l.P.in.cstride < 0 || l.P.in.estride < 0) return;
+  }
   if (l.st == ST_PLAIN || l.st == ST_TW) {
     const long long sp = tile_span_bytes(l.P.out, l.P.G, e->tpb, l.P.Lfft, esz);
     if (sp < 0 || sp >= lim) return;

@@ -375,6 +375,89 @@ __device__ __forceinline__ void epi_store(const KParams& P, void* dst, const cx<
   }
 }
 
+// DCT-II post-processing (dct1d.rs:266-287) from the final LDS image of a
+// tile of half length L (N = 2L): thread u of transform t takes the pairs
+// i = u + r * TPT, i <= L/2, and writes bins i, N-i, L-i and L+i; tables and
+// the row are addressed by 32-bit offsets.
+template <typename T, int L, int TPT, int TPB>
+__device__ __forceinline__ void epi_dct2(const KParams& P, void* dst, const cx<T>* lds, int LP, int g0, int nt, T s) {
+  constexpr int N = 2 * L, h = L, q2 = L / 2;  // N % 4 == 0 when L is even
+  const int tid = threadIdx.x;
+  const int t = tid / TPT, u = tid % TPT;
+  if (t >= nt) return;
+  const cx<T>* row = lds + t * LP;
+  const bool cjo = P.conj_out != 0;
+  auto Z = [&](int k) {
+    const cx<T> z = row[lx(k)];
+    return cjo ? cj(z) : z;
+  };
+  const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
+  const cx<T>* om = reinterpret_cast<const cx<T>*>(P.tab_b);
+  T* ret = reinterpret_cast<T*>(dst) + (long long)(g0 + t) * P.out.bdist;
+  constexpr unsigned CE = sizeof(cx<T>), RE = sizeof(T);
+  constexpr int NR = cdiv(q2 + 1, TPT);
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int i = u + r * TPT;
+    if (r * TPT + TPT - 1 <= q2 || i <= q2) {
+      if (i == 0) {
+        const cx<T> z = Z(0);
+        ret[0] = (z.re + z.im) * s;
+        ret[h] = (z.re - z.im) * om[h].re * s;
+      } else if (i < q2) {
+        const cx<T> zi = Z(i);
+        const cx<T> wc = cj(Z(h - i));
+        const cx<T> x = ld_off<cx<T>>(co, (unsigned)i * CE) * (zi - wc);
+        const cx<T> jj = cj(zi - x) * ld_off<cx<T>>(om, (unsigned)i * CE);
+        const cx<T> kk = (wc + x) * ld_off<cx<T>>(om, (unsigned)(h - i) * CE);
+        st_off<T>(ret, (unsigned)i * RE, jj.re * s);
+        st_off<T>(ret, (unsigned)(N - i) * RE, jj.im * s);
+        st_off<T>(ret, (unsigned)(h - i) * RE, kk.re * s);
+        st_off<T>(ret, (unsigned)(h + i) * RE, kk.im * s);
+      } else {  // i == q2
+        const cx<T> x = Z(q2) * om[q2];
+        ret[q2] = x.re * s;
+        ret[N - q2] = x.im * s;
+      }
+    }
+  }
+}
+
+// First-pass gather through the generic load operator (any LD, per element).
+template <typename T, int L, int TPT, int TPB, bool COL, int LD, int MODE, class RS>
+__device__ __forceinline__ void gather_generic(const KParams& P, const void* src, int g0,
+                                               cx<T> (&x)[Plan<RS, L, TPT>::EMAX()]) {
+  using PL = Plan<RS, L, TPT>;
+  constexpr bool BLUE = MODE == 1;
+  const int tid = threadIdx.x;
+  const int t = COL ? tid % TPB : tid / TPT;
+  const int u = COL ? tid / TPB : tid % TPT;
+  const int g = g0 + t;
+  const bool valid = g < P.G;
+  const cx<T>* chirp = reinterpret_cast<const cx<T>*>(P.chirp);
+  constexpr int R0 = PL::R(0), NB0 = PL::NB(0), B0 = PL::B(0);
+  const Loader<T, LD> ld = make_loader<T, LD>(P, src, valid ? g : 0);
+#pragma unroll
+  for (int b = 0; b < B0; b++) {
+    const int i = u + b * TPT;
+    if (PL::FULL(0) || i < NB0) {
+#pragma unroll
+      for (int j = 0; j < R0; j++) {
+        const int n = i + j * NB0;
+        cx<T> v = mk(T(0), T(0));
+        if (valid && (!BLUE || n < P.L)) {
+          v = load_elem<T, LD>(P, ld, n);
+          if constexpr (LD != LD_FS_CHIRP) {
+            if (P.conj_in) v = cj(v);
+          }
+          if constexpr (BLUE) v = v * chirp[n];
+        }
+        x[b * R0 + j] = v;
+      }
+    }
+  }
+}
+
 // First-pass gather of one tile: element i + j*L/R0 of butterfly i = u + b*TPT
 // of transform g0 + t.
 template <typename T, int L, int TPT, int TPB, bool COL, int LD, int MODE, class RS>
@@ -417,27 +500,66 @@ __device__ __forceinline__ void sfft_gather(const KParams& P, const void* src, i
         }
       }
     }
-  } else {
-    const Loader<T, LD> ld = make_loader<T, LD>(P, src, valid ? g : 0);
+  } else if constexpr (LD == LD_FS_CHIRP) {
+    // four-step Bluestein columns: element n of column c is sequence index
+    // s = c * cstride + n * estride of its transform; x[s] * chirp[s] for
+    // s < N, zero padding beyond (chirpz.rs:97-113, the inverse conjugating
+    // the input first). 32-bit sequence offsets; the chirp is read from a
+    // uniform base.
+    int c;
+    const long long base = io_base(P.in, valid ? g : g0, &c);
+    const long long seq0 = (long long)c * P.in.cstride;
+    const cx<T>* row = reinterpret_cast<const cx<T>*>(src) + (base - seq0);
+    const unsigned es = (unsigned)P.in.estride;
+    const unsigned s0 = (unsigned)seq0 + (unsigned)u * es;
+    const unsigned N = (unsigned)P.N;
+    const bool cji = P.conj_in != 0;
 #pragma unroll
     for (int b = 0; b < B0; b++) {
       const int i = u + b * TPT;
       if (PL::FULL(0) || i < NB0) {
 #pragma unroll
         for (int j = 0; j < R0; j++) {
-          const int n = i + j * NB0;
+          const unsigned sq = s0 + (unsigned)(b * TPT + j * NB0) * es;
           cx<T> v = mk(T(0), T(0));
-          if (valid && (!BLUE || n < P.L)) {
-            v = load_elem<T, LD>(P, ld, n);
-            if constexpr (LD != LD_FS_CHIRP) {
-              if (P.conj_in) v = cj(v);
-            }
-            if constexpr (BLUE) v = v * chirp[n];
+          if (sq < N) {
+            v = ld_off<cx<T>>(row, sq * ESZ);
+            if (cji) v = cj(v);
+            v = v * ld_off<cx<T>>(chirp, sq * ESZ);
           }
           x[b * R0 + j] = v;
         }
       }
     }
+  } else if constexpr (LD == LD_DCT2 && !COL && !BLUE && B0 == 1 && NB0 == TPT && R0 % 2 == 0 &&
+                       (TPT & (TPT - 1)) == 0 && TPT <= 64) {
+    // DCT-II pack (dct1d.rs:245-263) for an even half length L = N/2 whose
+    // first pass gives thread u the elements n = u + j * TPT: element n < N/4
+    // is (x[4n], x[4n+2]) and element L-1-n is (x[4n+3], x[4n+1]), both from
+    // the same four reals. Lane u reads the four reals of its slots j < R0/2
+    // (two contiguous complex-sized loads per slot), keeps element n and hands
+    // element L-1-n -- slot R0-1-j of lane TPT-1-u, the mirror lane of the
+    // same transform -- across with one lane shuffle per real.
+    if ((reinterpret_cast<uintptr_t>(src) & (sizeof(cx<T>) - 1)) == 0 && (P.in.bdist & 1) == 0) {  // uniform
+      const T* xr = reinterpret_cast<const T*>(src) + (long long)(valid ? g : g0) * P.in.bdist;
+      const T o = (T)P.ortho;
+      const bool cji = P.conj_in != 0;
+#pragma unroll
+      for (int j = 0; j < R0 / 2; j++) {
+        const unsigned n4 = 4u * (unsigned)(u + j * TPT);
+        const cx<T> lo = ld_off<cx<T>>(xr, n4 * (unsigned)sizeof(T));
+        const cx<T> hi = ld_off<cx<T>>(xr, (n4 + 2) * (unsigned)sizeof(T));
+        cx<T> mine = mk(lo.re, hi.re);
+        if (j == 0 && u == 0) mine.re = lo.re * o;  // element 0 carries the ortho factor
+        const cx<T> other = mk(__shfl_xor(hi.im, TPT - 1), __shfl_xor(lo.im, TPT - 1));
+        x[j] = cji ? cj(mine) : mine;
+        x[R0 - 1 - j] = cji ? cj(other) : other;
+      }
+    } else {
+      gather_generic<T, L, TPT, TPB, COL, LD, MODE, RS>(P, src, g0, x);
+    }
+  } else {
+    gather_generic<T, L, TPT, TPB, COL, LD, MODE, RS>(P, src, g0, x);
   }
 }
 
@@ -575,6 +697,8 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
         epi_store<T, L, TPT, TPB, ST, true>(P, dst, img, LP, g0, nt, (T)scale);
       else
         epi_store<T, L, TPT, TPB, ST, false>(P, dst, img, LP, g0, nt, (T)scale);
+    } else if constexpr (ST == ST_DCT2 && L % 2 == 0) {
+      epi_dct2<T, L, TPT, TPB>(P, dst, img, LP, g0, nt, (T)scale);
     } else {
       // the planner sets P.Lp == LP for specialised launches
       const TMap m = make_map(TPB);
