@@ -423,6 +423,35 @@ __device__ __forceinline__ void epi_dct2(const KParams& P, void* dst, const cx<T
   }
 }
 
+// DCT-III output (dct1d.rs:395-407) from the final LDS image, half length L
+// even: pair i < L/2 writes ret[4i .. 4i+3] = (a.re, b.im, a.im, b.re) with
+// a = Z(i), b = Z(L-1-i), as two aligned two-real stores.
+template <typename T, int L, int TPT, int TPB>
+__device__ __forceinline__ void epi_dct3(const KParams& P, void* dst, const cx<T>* lds, int LP, int g0, int nt, T s) {
+  constexpr int h = L, q2 = L / 2;
+  const int tid = threadIdx.x;
+  const int t = tid / TPT, u = tid % TPT;
+  if (t >= nt) return;
+  const cx<T>* row = lds + t * LP;
+  const bool cjo = P.conj_out != 0;
+  T* ret = reinterpret_cast<T*>(dst) + (long long)(g0 + t) * P.out.bdist;
+  constexpr unsigned RE = sizeof(T);
+  constexpr int NR = cdiv(q2, TPT);
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int i = u + r * TPT;
+    if (q2 % TPT == 0 || i < q2) {
+      cx<T> a = row[lx(i)], b = row[lx(h - 1 - i)];
+      if (cjo) {
+        a = cj(a);
+        b = cj(b);
+      }
+      st_off<cx<T>>(ret, (unsigned)(4 * i) * RE, mk(a.re * s, b.im * s));
+      st_off<cx<T>>(ret, (unsigned)(4 * i + 2) * RE, mk(a.im * s, b.re * s));
+    }
+  }
+}
+
 // First-pass gather through the generic load operator (any LD, per element).
 template <typename T, int L, int TPT, int TPB, bool COL, int LD, int MODE, class RS>
 __device__ __forceinline__ void gather_generic(const KParams& P, const void* src, int g0,
@@ -553,6 +582,90 @@ __device__ __forceinline__ void sfft_gather(const KParams& P, const void* src, i
         if (j == 0 && u == 0) mine.re = lo.re * o;  // element 0 carries the ortho factor
         const cx<T> other = mk(__shfl_xor(hi.im, TPT - 1), __shfl_xor(lo.im, TPT - 1));
         x[j] = cji ? cj(mine) : mine;
+        x[R0 - 1 - j] = cji ? cj(other) : other;
+      }
+    } else {
+      gather_generic<T, L, TPT, TPB, COL, LD, MODE, RS>(P, src, g0, x);
+    }
+  } else if constexpr (LD == LD_DCT3 && !COL && !BLUE && B0 == 1 && NB0 == TPT && R0 % 2 == 0 &&
+                       (TPT & (TPT - 1)) == 0 && TPT <= 64 && L % 2 == 0) {
+    // DCT-III pre-processing (dct1d.rs:367-391), half length L = N/2 even,
+    // q = L/2: elements n and L-n (0 < n < q) come from the same i = n, so
+    // lane u computes both for its slots j < R0/2 (n = u + j * TPT < q) and
+    // element L-n goes to the lane that holds it: lane TPT-u at slot
+    // R0-1-j, or lane 0 itself at slot R0-j when u = 0. Element 0 and
+    // element q have their own formulas.
+    const T* xr = reinterpret_cast<const T*>(src) + (long long)(valid ? g : g0) * P.in.bdist;
+    const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
+    const cx<T>* om = reinterpret_cast<const cx<T>*>(P.tab_b);
+    constexpr int N = 2 * L, h = L, q = L / 2;
+    constexpr unsigned RE = sizeof(T), CE = sizeof(cx<T>);
+    const bool cji = P.conj_in != 0;
+    const int src_lane = (TPT - u) & (TPT - 1);
+    cx<T> mine[R0 / 2], other[R0 / 2];
+#pragma unroll
+    for (int j = 0; j < R0 / 2; j++) {
+      const int i = u + j * TPT;
+      const int ni = i == 0 ? 0 : N - i;  // element 0 has its own formula; keep its loads in the row
+      const cx<T> a = mk(ld_off<T>(xr, (unsigned)i * RE), ld_off<T>(xr, (unsigned)ni * RE));
+      const cx<T> bb = mk(ld_off<T>(xr, (unsigned)(h - i) * RE), ld_off<T>(xr, (unsigned)(h + i) * RE));
+      const cx<T> jv = cj(a * ld_off<cx<T>>(om, (unsigned)i * CE));
+      const cx<T> kv = bb * ld_off<cx<T>>(om, (unsigned)(h - i) * CE);
+      const cx<T> xx = ld_off<cx<T>>(co, (unsigned)i * CE) * (jv - kv);
+      mine[j] = jv - xx;
+      other[j] = cj(kv) + cj(xx);
+    }
+    if (u == 0) {  // element 0
+      const T z = xr[0] * (T)P.ortho;
+      const T hv = xr[h] * om[h].re;
+      mine[0] = mk(z + hv, z - hv);
+    }
+#pragma unroll
+    for (int j = 0; j < R0 / 2; j++) {
+      x[j] = cji ? cj(mine[j]) : mine[j];
+      const int J = R0 - 1 - j;  // slots R0/2 .. R0-1 receive element L - n'
+      const cx<T> got = mk(__shfl(other[j].re, src_lane, TPT), __shfl(other[j].im, src_lane, TPT));
+      cx<T> v = got;
+      if (u == 0) {
+        if (J == R0 / 2)  // element q
+          v = mk(xr[q], xr[N - q]) * om[q];
+        else
+          v = other[R0 - J];
+      }
+      x[J] = cji ? cj(v) : v;
+    }
+  } else if constexpr (LD == LD_MDCT && !COL && !BLUE && B0 == 1 && NB0 == TPT && R0 % 2 == 0 &&
+                       (TPT & (TPT - 1)) == 0 && TPT <= 64) {
+    // MDCT fold (mdct1d.rs:206-230), frame length N = 4L: element n < L/2
+    // reads the reals at q3-1-m, q3+m, q+m, q-1-m (m = 2n, q = L, q3 = 3L)
+    // and element L-1-n their neighbours q+1+m, q-2-m, q3-2-m, q3+1+m, so
+    // four aligned pairs of reals (and of window values) serve both. Lane u
+    // folds both elements of its slots j < R0/2 and hands element L-1-n to
+    // the mirror lane TPT-1-u (slot R0-1-j) with one shuffle per real.
+    if ((reinterpret_cast<uintptr_t>(src) & (sizeof(cx<T>) - 1)) == 0 && (P.in.bdist & 1) == 0) {  // uniform
+      const T* xr = reinterpret_cast<const T*>(src) + (long long)(valid ? g : g0) * P.in.bdist;
+      const T* wr = reinterpret_cast<const T*>(P.tab_b);
+      const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
+      constexpr int q = L, q3 = 3 * L;
+      constexpr unsigned RE = sizeof(T), CE = sizeof(cx<T>);
+      const bool cji = P.conj_in != 0;
+#pragma unroll
+      for (int j = 0; j < R0 / 2; j++) {
+        const int n = u + j * TPT, m = 2 * n;
+        const cx<T> p1 = ld_off<cx<T>>(xr, (unsigned)(q + m) * RE);           // (A3, B1)
+        const cx<T> p2 = ld_off<cx<T>>(xr, (unsigned)(q - 2 - m) * RE);       // (B2, A4)
+        const cx<T> p3 = ld_off<cx<T>>(xr, (unsigned)(q3 - 2 - m) * RE);      // (B3, A1)
+        const cx<T> p4 = ld_off<cx<T>>(xr, (unsigned)(q3 + m) * RE);          // (A2, B4)
+        const cx<T> w1 = ld_off<cx<T>>(wr, (unsigned)(q + m) * RE);
+        const cx<T> w2 = ld_off<cx<T>>(wr, (unsigned)(q - 2 - m) * RE);
+        const cx<T> w3 = ld_off<cx<T>>(wr, (unsigned)(q3 - 2 - m) * RE);
+        const cx<T> w4 = ld_off<cx<T>>(wr, (unsigned)(q3 + m) * RE);
+        cx<T> va = mk(p3.im * w3.im + p4.re * w4.re, p1.re * w1.re - p2.im * w2.im);
+        cx<T> vb = mk(p1.im * w1.im - p2.re * w2.re, p3.re * w3.re + p4.im * w4.im);
+        va = va * ld_off<cx<T>>(tw, (unsigned)n * CE);
+        vb = vb * ld_off<cx<T>>(tw, (unsigned)(L - 1 - n) * CE);
+        const cx<T> other = mk(__shfl_xor(vb.re, TPT - 1), __shfl_xor(vb.im, TPT - 1));
+        x[j] = cji ? cj(va) : va;
         x[R0 - 1 - j] = cji ? cj(other) : other;
       }
     } else {
@@ -699,6 +812,13 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
         epi_store<T, L, TPT, TPB, ST, false>(P, dst, img, LP, g0, nt, (T)scale);
     } else if constexpr (ST == ST_DCT2 && L % 2 == 0) {
       epi_dct2<T, L, TPT, TPB>(P, dst, img, LP, g0, nt, (T)scale);
+    } else if constexpr (ST == ST_DCT3 && L % 2 == 0) {
+      if ((reinterpret_cast<uintptr_t>(dst) & (sizeof(cx<T>) - 1)) == 0 && (P.out.bdist & 1) == 0) {
+        epi_dct3<T, L, TPT, TPB>(P, dst, img, LP, g0, nt, (T)scale);
+      } else {
+        const TMap m = make_map(TPB);
+        store_stage<T, ST>(P, m, dst, img, g0, nt, scale);
+      }
     } else {
       // the planner sets P.Lp == LP for specialised launches
       const TMap m = make_map(TPB);
@@ -730,6 +850,52 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
             }
           }
         }
+      }
+    } else if constexpr (ST == ST_MDCT && !COL && !BLUE && PL::FULL(NP - 1) && (TPT & (TPT - 1)) == 0 &&
+                         TPT <= 64) {
+      // MDCT output (mdct1d.rs:234-240): bin k gives ret[2k] = -w_k.re and
+      // ret[N/2-1-2k] = w_k.im; bins k and L-1-k fill the aligned pair
+      // ret[2k], ret[2k+1] = (-w_k.re, w_(L-1-k).im). Bin L-1-k sits in the
+      // mirror lane TPT-1-u at the mirrored slot; one shuffle brings its
+      // imaginary part over, and each lane stores one pair per bin.
+      const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
+      T* ret = reinterpret_cast<T*>(dst) + (long long)g * P.out.bdist;
+      constexpr unsigned RE = sizeof(T), CE = sizeof(cx<T>);
+      const bool cjo = P.conj_out != 0;
+      if ((reinterpret_cast<uintptr_t>(dst) & (sizeof(cx<T>) - 1)) == 0 && (P.out.bdist & 1) == 0) {  // uniform
+        T wim[BL * RL], wre[BL * RL];
+#pragma unroll
+        for (int b = 0; b < BL; b++)
+#pragma unroll
+          for (int j = 0; j < RL; j++) {
+            const int k = u + b * TPT + j * PLAST;
+            cx<T> v = x[b * RL + j];
+            if (cjo) v = cj(v);
+            const cx<T> w = v * scl(ld_off<cx<T>>(tw, (unsigned)k * CE), s);
+            wre[b * RL + j] = w.re;
+            wim[b * RL + j] = w.im;
+          }
+#pragma unroll
+        for (int b = 0; b < BL; b++)
+#pragma unroll
+          for (int j = 0; j < RL; j++) {
+            const int k = u + b * TPT + j * PLAST;
+            const T pim = __shfl_xor(wim[(BL - 1 - b) * RL + (RL - 1 - j)], TPT - 1);
+            st_off<cx<T>>(ret, (unsigned)(2 * k) * RE, mk(-wre[b * RL + j], pim));
+          }
+      } else {
+        const int h = (int)(P.N >> 1);
+#pragma unroll
+        for (int b = 0; b < BL; b++)
+#pragma unroll
+          for (int j = 0; j < RL; j++) {
+            const int k = u + b * TPT + j * PLAST;
+            cx<T> v = x[b * RL + j];
+            if (cjo) v = cj(v);
+            const cx<T> w = v * scl(tw[k], s);
+            ret[2 * k] = -w.re;
+            ret[h - 1 - 2 * k] = w.im;
+          }
       }
     } else {
       int c = 0;

@@ -287,3 +287,44 @@ def test_headline_full_batch(dtype):
         y = yd.cpu().numpy()
         for b in (0, 1, 7, 8, 255, 256, 510, 511):
             check(y[b], oracle.c2c(x[b].astype(np.complex128), m), f64, n)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_operator_paths_config_size(dtype):
+    # DCT-II/III (N = 1024, with and without is_ortho) and MDCT (N = 2048,
+    # hop 1024) at their benchmark sizes on device buffers, once aligned and
+    # once offset by one real (the kernels then take their per-element path):
+    # both against the oracle
+    import torch
+    from chfft_amd import Plan
+    f64 = dtype == np.float64
+    tdt = torch.float64 if f64 else torch.float32
+    r = rng(60)
+    n, batch = 1024, 64
+    x = (2 * r.random((batch, n)) - 1).astype(dtype)
+    for t in (2, 3):
+        for ortho in (False, True):
+            p = Plan("dct", f64, batch, t, n, ortho)
+            ref = oracle.dct(x.astype(np.float64), t, ortho, "forward")
+            for off in (0, 1):
+                xd = torch.zeros(batch * n + 1, dtype=tdt, device="cuda")
+                xd[off:off + batch * n] = torch.from_numpy(x.reshape(-1)).cuda()
+                yd = torch.zeros(batch * n + 1, dtype=tdt, device="cuda")
+                p.exec_ptr(xd[off:].data_ptr(), yd[off:].data_ptr(), "forward",
+                           torch.cuda.current_stream().cuda_stream)
+                torch.cuda.synchronize()
+                got = yd[off:off + batch * n].cpu().numpy().reshape(batch, n)
+                check(got, ref, f64, n)
+    nm, hop, frames = 2048, 1024, 40
+    sig = r.standard_normal(hop * (frames - 1) + nm).astype(dtype)
+    win = oracle.window(nm, "sine")
+    p = Plan("mdct", f64, frames, nm, win, hop)
+    ref = oracle.mdct(sig.astype(np.float64), win, "forward", n=nm, hop=hop, frames=frames)
+    for off in (0, 1):
+        xd = torch.zeros(len(sig) + 1, dtype=tdt, device="cuda")
+        xd[off:off + len(sig)] = torch.from_numpy(sig).cuda()
+        yd = torch.zeros(frames * nm // 2 + 1, dtype=tdt, device="cuda")
+        p.exec_ptr(xd[off:].data_ptr(), yd[off:].data_ptr(), "forward", torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        got = yd[off:off + frames * nm // 2].cpu().numpy().reshape(frames, nm // 2)
+        check(got, ref, f64, nm)
