@@ -185,6 +185,49 @@ __device__ __forceinline__ void dft16(cx<T>* x) {
   }
 }
 
+// Prime-factor (Good-Thomas) DFTs for R = R1 * R2 with coprime factors: no
+// internal twiddles. Input n = (R2 n1 + R1 n2) mod R goes to row n1, column
+// n2; R1-point DFTs along each column, R2-point DFTs along each row, and
+// output (k1, k2) is X[k] with k = k1 mod R1, k = k2 mod R2 (CRT):
+// k = (A k1 + B k2) mod R.
+template <typename T, int R1, int R2, int A, int B, class D1, class D2>
+__device__ __forceinline__ void dft_pfa(cx<T>* x) {
+  constexpr int R = R1 * R2;
+  cx<T> m[R2][R1];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; n2++)
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) m[n2][n1] = x[(R2 * n1 + R1 * n2) % R];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; n2++) D1::template run<T>(m[n2]);
+#pragma unroll
+  for (int k1 = 0; k1 < R1; k1++) {
+    cx<T> c[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; n2++) c[n2] = m[n2][k1];
+    D2::template run<T>(c);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) x[(A * k1 + B * k2) % R] = c[k2];
+  }
+}
+struct Dft3 {
+  template <typename T>
+  __device__ __forceinline__ static void run(cx<T>* x) { dft3(x); }
+};
+struct Dft4 {
+  template <typename T>
+  __device__ __forceinline__ static void run(cx<T>* x) { dft4(x); }
+};
+struct Dft5 {
+  template <typename T>
+  __device__ __forceinline__ static void run(cx<T>* x) { dft5(x); }
+};
+// 12 = 4 x 3: k = 9 k1 + 4 k2 (mod 12); 15 = 3 x 5: k = 10 k1 + 6 k2 (mod 15)
+template <typename T>
+__device__ __forceinline__ void dft12(cx<T>* x) { dft_pfa<T, 4, 3, 9, 4, Dft4, Dft3>(x); }
+template <typename T>
+__device__ __forceinline__ void dft15(cx<T>* x) { dft_pfa<T, 3, 5, 10, 6, Dft3, Dft5>(x); }
+
 template <typename T, int R>
 __device__ __forceinline__ void dft(cx<T>* x) {
   if constexpr (R == 2) dft2(x);
@@ -193,6 +236,8 @@ __device__ __forceinline__ void dft(cx<T>* x) {
   else if constexpr (R == 5) dft5(x);
   else if constexpr (R == 7) dft7(x);
   else if constexpr (R == 8) dft8(x);
+  else if constexpr (R == 12) dft12(x);
+  else if constexpr (R == 15) dft15(x);
   else if constexpr (R == 16) dft16(x);
 }
 

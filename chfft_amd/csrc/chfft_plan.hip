@@ -56,6 +56,7 @@ struct SfftEntry {
   void (*fn)(KParams);
 };
 const SfftEntry* sfft_find(bool f64, int L, int ld, int st, int mode, bool col);
+bool sfft_has_schedule(bool f64, int L, const int* rad, int nrad);
 hipError_t sfft_launch(const SfftEntry* e, const KParams& P, dim3 grid, size_t lds, hipStream_t st);
 hipError_t sfft_set_lds_limits();
 size_t sfft_lds_bytes(const SfftEntry* e);
@@ -255,8 +256,11 @@ size_t next_pow2(size_t n) {
 }
 
 // Radix schedule for a 7-smooth L: as few passes as the radix set allows
-// (16, 8, 4, 2 for the power of two; 3, 5, 7 for the odd part).
-std::vector<int> radices_for(size_t L) {
+// (16, 8, 4, 2 for the power of two; 3, 5, 7 for the odd part). With `pfa`
+// each 3 x 5 pair becomes one prime-factor radix-15 pass and a lone 4 with a
+// 3 one radix-12 pass (180 = 12 x 15, 240 = 16 x 15): a schedule only the
+// length-specialised kernels run (the runtime-radix kernels stop at 16).
+std::vector<int> radices_for(size_t L, bool pfa = false) {
   std::vector<int> r;
   int a = 0;
   while (L % 2 == 0) {
@@ -276,11 +280,30 @@ std::vector<int> radices_for(size_t L) {
     r.push_back(8);
   }
   for (int i = 0; i < k16; i++) r.insert(r.begin(), 16);
-  for (int p : {5, 3, 7})
-    while (L % p == 0) {
-      L /= p;
-      r.push_back(p);
-    }
+  int n3 = 0, n5 = 0, n7 = 0;
+  while (L % 5 == 0) {
+    L /= 5;
+    n5++;
+  }
+  while (L % 3 == 0) {
+    L /= 3;
+    n3++;
+  }
+  while (L % 7 == 0) {
+    L /= 7;
+    n7++;
+  }
+  const int n15 = pfa ? std::min(n3, n5) : 0;
+  n3 -= n15;
+  n5 -= n15;
+  if (pfa && n3 > 0 && !r.empty() && r.back() == 4 && (r.size() == 1 || r[r.size() - 2] != 8)) {
+    r.back() = 12;  // the lone 4 (not the 8 x 4 tail) joins a 3
+    n3--;
+  }
+  for (int i = 0; i < n15; i++) r.push_back(15);
+  for (int i = 0; i < n5; i++) r.push_back(5);
+  for (int i = 0; i < n3; i++) r.push_back(3);
+  for (int i = 0; i < n7; i++) r.push_back(7);
   return r;
 }
 
@@ -375,8 +398,8 @@ struct Inner {
   DevBuf tw, chirp, kspec;
 };
 
-int build_seq(size_t L, bool f64, SeqDesc& seq, DevBuf& tw) {
-  std::vector<int> rad = radices_for(L);
+int build_seq(size_t L, bool f64, SeqDesc& seq, DevBuf& tw, bool pfa = false) {
+  std::vector<int> rad = radices_for(L, pfa);
   if ((int)rad.size() > kMaxPass) return CHFFT_ERR_UNSUPPORTED;
   seq.L = (int)L;
   seq.npass = (int)rad.size();
@@ -540,16 +563,25 @@ long long tile_span_bytes(const IoDesc& d, long long G, int tpb, long long len, 
   return (worst + (len - 1) * d.estride + 1) * (long long)esz;
 }
 
+bool same_schedule(const SfftEntry* e, const SeqDesc& s) {
+  if (e->nrad != s.npass) return false;
+  for (int q = 0; q < e->nrad; q++)
+    if (e->rad[q] != s.pass[q].radix) return false;
+  return true;
+}
+
 // Use a length-specialised kernel for a fused launch when one is registered
-// for this length, operator pair and radix schedule.
-void try_specialise(Launch& l, bool f64) {
+// for this length, operator pair and radix schedule: the launch's own
+// schedule, or `alt` (the same length under prime-factor radices, with its
+// own twiddle tables), which then replaces it.
+void try_specialise(Launch& l, bool f64, const SeqDesc* alt = nullptr) {
   if (l.kind != 0) return;
   const int mode = l.P.has_mid ? (l.P.has_chirp ? 1 : 2) : 0;
   const bool col = l.P.colmajor_ld != 0;
   const SfftEntry* e = sfft_find(f64, l.P.Lfft, l.ld, l.st, mode, col);
-  if (!e || e->nrad != l.P.seq.npass) return;
-  for (int q = 0; q < e->nrad; q++)
-    if (e->rad[q] != l.P.seq.pass[q].radix) return;
+  if (!e) return;
+  const bool use_alt = !same_schedule(e, l.P.seq);
+  if (use_alt && !(alt && alt->L == l.P.Lfft && same_schedule(e, *alt))) return;
   // a direct (non-epilogue) store uses the load's thread layout
   if (!e->epi && l.P.colmajor_st != l.P.colmajor_ld) return;
   // plain loads and plain / twiddled stores address each tile by 32-bit
@@ -576,12 +608,18 @@ void try_specialise(Launch& l, bool f64) {
   }
   l.kind = 3;
   l.sf = e;
+  if (use_alt) l.P.seq = *alt;
   l.P.tpb = e->tpb;
   l.P.Lp = sfft_lds_stride(e);
   l.grid = dim3((unsigned)((l.P.G + e->tpb - 1) / e->tpb));
   l.block = dim3((unsigned)(e->tpt * e->tpb));
   l.lds = sfft_lds_bytes(e);
   l.what = "specialised " + l.what;
+  if (use_alt) {
+    l.what += " run as radix[";
+    for (int q = 0; q < alt->npass; q++) l.what += (q ? "," : "") + std::to_string(alt->pass[q].radix);
+    l.what += "]";
+  }
 }
 
 IoDesc io(long long bdist, long long cstride, long long estride, int ncols) {
@@ -609,7 +647,8 @@ struct C2CEngine {
   Inner inner;   // mode 1
   size_t n1 = 0, n2 = 0;
   SeqDesc s1{}, s2{};
-  DevBuf tw1, tw2, twN, chirp, kspec;
+  SeqDesc x1{}, x2{};  // prime-factor schedules of n1, n2 (L = 0: none registered)
+  DevBuf tw1, tw2, twx1, twx2, twN, chirp, kspec;
   size_t M = 0;
 
   int init(size_t len, bool use_f64) {
@@ -640,6 +679,8 @@ struct C2CEngine {
     int rc;
     if ((rc = build_seq(n1, f64, s1, tw1))) return rc;
     if ((rc = build_seq(n2, f64, s2, tw2))) return rc;
+    if ((rc = build_pfa(n1, x1, twx1))) return rc;
+    if ((rc = build_pfa(n2, x2, twx2))) return rc;
     if ((rc = upload(fourstep_table(n1, n2), f64, twN))) return rc;
     if (mode == 3) {
       if ((rc = upload(chirp_table(len), f64, chirp))) return rc;
@@ -656,6 +697,15 @@ struct C2CEngine {
       if ((rc = upload(Kt, f64, kspec))) return rc;
     }
     return CHFFT_OK;
+  }
+
+  // The prime-factor schedule of L when it differs from the plain one and a
+  // specialised kernel runs it.
+  int build_pfa(size_t L, SeqDesc& seq, DevBuf& tw) const {
+    seq = SeqDesc{};
+    const std::vector<int> rp = radices_for(L, true);
+    if (rp == radices_for(L) || !sfft_has_schedule(f64, (int)L, rp.data(), (int)rp.size())) return CHFFT_OK;
+    return build_seq(L, f64, seq, tw, true);
   }
 
   // scratch of the split (multi-launch) four-step
@@ -718,7 +768,7 @@ struct C2CEngine {
       a.P.twc = 1;
       finish(a, choose_geometry((int)n1, a.P.G, true, f64), src, S_SCRATCH, 0);
       a.what = "four-step columns " + seq_str(s1) + " + twiddle";
-      try_specialise(a, f64);
+      try_specialise(a, f64, &x1);
     out.push_back(a);
       // K2: rows (length n2) -> out[k1 + n1*k2]
       Launch b{};
@@ -732,7 +782,7 @@ struct C2CEngine {
       b.P.conj_out = ci;
       finish(b, choose_geometry((int)n2, b.P.G, true, f64), S_SCRATCH, dst, 1);
       b.what = "four-step rows " + seq_str(s2) + " + transpose";
-      try_specialise(b, f64);
+      try_specialise(b, f64, &x2);
     out.push_back(b);
       return;
     }
@@ -754,7 +804,7 @@ struct C2CEngine {
     a.P.twc = 1;
     finish(a, choose_geometry((int)n1, a.P.G, true, f64), src, S_SCRATCH, 0);
     a.what = "bluestein N=" + std::to_string(n) + " chirp + columns " + seq_str(s1) + " + twiddle";
-    try_specialise(a, f64);
+    try_specialise(a, f64, &x1);
     out.push_back(a);
     Launch b{};
     b.ld = LD_PLAIN;
@@ -773,7 +823,7 @@ struct C2CEngine {
     b.P.twc = (int)n2;
     finish(b, choose_geometry((int)n2, b.P.G, false, f64), S_SCRATCH, S_SCRATCH, 0);
     b.what = "bluestein rows " + seq_str(s2) + " x spectrum, inverse rows + twiddle";
-    try_specialise(b, f64);
+    try_specialise(b, f64, &x2);
     out.push_back(b);
     Launch c{};
     c.ld = LD_PLAIN;
@@ -788,7 +838,7 @@ struct C2CEngine {
     c.P.conj_out = ci;
     finish(c, choose_geometry((int)n1, c.P.G, true, f64), S_SCRATCH, dst, 1);
     c.what = "bluestein inverse columns " + seq_str(s1) + " + de-chirp";
-    try_specialise(c, f64);
+    try_specialise(c, f64, &x1);
     out.push_back(c);
   }
 

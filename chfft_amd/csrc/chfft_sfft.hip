@@ -75,7 +75,7 @@ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 #define CHFFT_TW_AHEAD_BYTES 128
 #endif
 #ifndef CHFFT_TW_EARLY_BYTES
-#define CHFFT_TW_EARLY_BYTES 96
+#define CHFFT_TW_EARLY_BYTES 128
 #endif
 // Per-pass twiddle tables up to this size (per precision) are copied into
 // LDS once per tile and read from there (shorter latency than the L2, and
@@ -1048,13 +1048,10 @@ static const SfftEntry kSfft[] = {
     // 8192: fused Bluestein for N <= 4096 (N = 4093 and smaller primes), CFft1D
     SF_BOTH(8192, 512, 1, 1, false, LD_PLAIN, ST_PLAIN, 1, false, 4, 16, 16, 8, 4),
     SF_BOTH(8192, 512, 1, 1, false, LD_PLAIN, ST_PLAIN, 0, false, 4, 16, 16, 8, 4),
-    // 43200 = 200 x 216 four-step: 16-column tiles + twiddle, 16-row tiles +
-    // transposed store (LDS epilogue)
-    SF_BOTH(200, 16, 16, 16, true, LD_PLAIN, ST_TW, 0, false, 3, 8, 5, 5),
-    SF_BOTH(216, 16, 16, 16, false, LD_PLAIN, ST_PLAIN, 0, true, 4, 8, 3, 3, 3),
-    // 43200 = 180 x 240 (rows of the column pass 128-byte aligned)
-    SF_BOTH(180, 16, 16, 8, true, LD_PLAIN, ST_TW, 0, false, 4, 4, 5, 3, 3),
-    SF_BOTH(240, 16, 16, 8, false, LD_PLAIN, ST_PLAIN, 0, true, 3, 16, 5, 3),
+    // 43200 = 180 x 240 (rows of the column pass 128-byte aligned), two
+    // passes each: 180 = 12 x 15, 240 = 16 x 15 (prime-factor radices)
+    SF_BOTH(180, 16, 16, 8, true, LD_PLAIN, ST_TW, 0, false, 2, 12, 15),
+    SF_BOTH(240, 16, 16, 8, false, LD_PLAIN, ST_PLAIN, 0, true, 2, 16, 15),
     // four-step Bluestein: chirp + columns, rows x spectrum + inverse rows,
     // inverse columns + de-chirp; M = 20480 = 128 x 160 (N = 10007),
     // M = 32768 = 128 x 256, M = 147456 = 384 x 384 (N = 65537)
@@ -1138,6 +1135,18 @@ const SfftEntry* sfft_find(bool f64, int L, int ld, int st, int mode, bool col) 
     if (e.f64 == (int)f64 && e.L == L && e.ld == ld && e.st == st && e.mode == mode && e.col == (int)col)
       return &e;
   return nullptr;
+}
+
+// Whether some registered kernel of this length runs exactly this radix
+// schedule.
+bool sfft_has_schedule(bool f64, int L, const int* rad, int nrad) {
+  for (const SfftEntry& e : kSfft) {
+    if (e.f64 != (int)f64 || e.L != L || e.nrad != nrad) continue;
+    bool same = true;
+    for (int q = 0; q < nrad; q++) same = same && e.rad[q] == rad[q];
+    if (same) return true;
+  }
+  return false;
 }
 
 hipError_t sfft_launch(const SfftEntry* e, const KParams& P, dim3 grid, size_t lds, hipStream_t st) {
