@@ -191,9 +191,11 @@ def headline(torch, args):
     r64, p64, b64, _ = c2c_case(torch, N, BATCH, True, args.steps, args.warmup, SEED)
     k32 = r32["kernel_ms"]
     dom = int(np.argmax(k32))
-    # each launch of the four-step reads and writes the whole batch once:
-    # algorithmic bytes per launch = 16 N per transform (FP32)
-    bytes_launch = 16.0 * N * BATCH
+    # the split four-step runs in equal chunks of transforms, two launches
+    # (columns, rows) per chunk; each launch reads and writes its chunk once:
+    # algorithmic bytes per launch = 16 N per transform of the chunk (FP32)
+    chunks = max(1, len(k32) // 2)
+    bytes_launch = 16.0 * N * BATCH / chunks
     achieved = bytes_launch / (k32[dom] * 1e6)
     cpu = None
     if not args.no_cpu:

@@ -384,6 +384,7 @@ struct Launch {
   dim3 grid, block;
   size_t lds;
   int src, dst;      // Slot
+  long long src_off = 0, dst_off = 0;  // bytes into the in / out slots (chunked launches)
   int scaled;        // receives the method's scale
   double scale_mul;  // extra factor (RFft backward: 2)
   std::string what;
@@ -563,6 +564,12 @@ long long tile_span_bytes(const IoDesc& d, long long G, int tpb, long long len, 
   return (worst + (len - 1) * d.estride + 1) * (long long)esz;
 }
 
+std::string seq_str(const SeqDesc& s) {
+  std::string r = "L=" + std::to_string(s.L) + " radix[";
+  for (int i = 0; i < s.npass; i++) r += (i ? "," : "") + std::to_string(s.pass[i].radix);
+  return r + "]";
+}
+
 bool same_schedule(const SfftEntry* e, const SeqDesc& s) {
   if (e->nrad != s.npass) return false;
   for (int q = 0; q < e->nrad; q++)
@@ -608,18 +615,18 @@ void try_specialise(Launch& l, bool f64, const SeqDesc* alt = nullptr) {
   }
   l.kind = 3;
   l.sf = e;
-  if (use_alt) l.P.seq = *alt;
   l.P.tpb = e->tpb;
   l.P.Lp = sfft_lds_stride(e);
   l.grid = dim3((unsigned)((l.P.G + e->tpb - 1) / e->tpb));
   l.block = dim3((unsigned)(e->tpt * e->tpb));
   l.lds = sfft_lds_bytes(e);
-  l.what = "specialised " + l.what;
   if (use_alt) {
-    l.what += " run as radix[";
-    for (int q = 0; q < alt->npass; q++) l.what += (q ? "," : "") + std::to_string(alt->pass[q].radix);
-    l.what += "]";
+    const std::string a = seq_str(*alt), o = seq_str(l.P.seq);
+    const size_t at = l.what.find(o);
+    if (at != std::string::npos) l.what.replace(at, o.size(), a);
+    l.P.seq = *alt;
   }
+  l.what = "specialised " + l.what;
 }
 
 IoDesc io(long long bdist, long long cstride, long long estride, int ncols) {
@@ -629,12 +636,6 @@ IoDesc io(long long bdist, long long cstride, long long estride, int ncols) {
   d.estride = estride;
   d.ncols = ncols;
   return d;
-}
-
-std::string seq_str(const SeqDesc& s) {
-  std::string r = "L=" + std::to_string(s.L) + " radix[";
-  for (int i = 0; i < s.npass; i++) r += (i ? "," : "") + std::to_string(s.pass[i].radix);
-  return r + "]";
 }
 
 }  // namespace
@@ -711,7 +712,7 @@ struct C2CEngine {
   // scratch of the split (multi-launch) four-step
   size_t scratch_elems(size_t batch) const {
     if (persistent_ok(batch)) return 0;
-    return (mode == 2 ? n : mode == 3 ? M : 0) * batch;
+    return (mode == 2 ? n : mode == 3 ? M : 0) * chunk_of(batch);
   }
 
   // Appends the launches for `batch` transforms; `inverse` selects the
@@ -751,6 +752,52 @@ struct C2CEngine {
       emit_persistent(out, batch, inverse, src, dst, dist_in, dist_out);
       return;
     }
+    // split launches, chunk by chunk through one scratch of chunk transforms
+    const size_t cb = chunk_of(batch);
+    const long long esz = f64 ? 16 : 8;
+    for (size_t c0 = 0; c0 < batch; c0 += cb) {
+      const size_t first = out.size();
+      emit_split(out, std::min(cb, batch - c0), inverse, src, dst, dist_in, dist_out);
+      for (size_t i = first; i < out.size(); i++) {
+        Launch& l = out[i];
+        if (l.src != S_SCRATCH) l.src_off = (long long)c0 * dist_in * esz;
+        if (l.dst != S_SCRATCH) l.dst_off = (long long)c0 * dist_out * esz;
+        if (cb < batch) l.what += " (transforms " + std::to_string(c0) + "+)";
+      }
+    }
+  }
+
+  // Transforms per chunk of the split four-step. A chunk's scratch (its
+  // intermediate spectra) is written by the column launch and read back by
+  // the row launch; a 7-smooth four-step whose scratch would exceed
+  // kChunkScratchBytes runs in equal chunks below it, so that the scratch
+  // stays in the 256 MB memory-side cache. Measured on MI355X, 43200 x 512
+  // with rotated 1 GB buffers: FP32 155 -> 142 us (2 chunks, 88 MB each;
+  // 4 chunks: 166 us), FP64 301 -> 286 us (4 chunks; 2 chunks of 177 MB:
+  // 325 us). The three-launch Bluestein four-step (N = 10007 x 512,
+  // 65537 x 64) ran 10-25 % slower in chunks and keeps the whole batch.
+  // CHFFT_FS_CHUNK overrides (transforms per chunk; -1: whole batch; 0:
+  // the rule above).
+  static constexpr size_t kChunkScratchBytes = (size_t)96 << 20;
+  size_t chunk_of(size_t batch) const {
+    size_t c = batch;
+    if (mode == 2) {
+      const size_t per = n * (f64 ? 16 : 8);
+      const size_t cmax = std::max<size_t>(1, kChunkScratchBytes / per);
+      const size_t k = (batch + cmax - 1) / cmax;
+      c = (batch + k - 1) / k;
+    }
+    if (const char* e = getenv("CHFFT_FS_CHUNK")) {
+      const long long v = atoll(e);
+      if (v > 0) c = (size_t)v;
+      if (v == -1) c = batch;
+    }
+    return std::max<size_t>(1, std::min(c, batch));
+  }
+
+  void emit_split(std::vector<Launch>& out, size_t batch, bool inverse, int src, int dst, long long dist_in,
+                  long long dist_out) const {
+    const int ci = inverse ? 1 : 0;
     if (mode == 2) {
       // K1: columns (length n1, stride n2) + twiddle W_n^(n2*k1) -> scratch
       Launch a{};
@@ -1475,6 +1522,8 @@ int chfft_exec_range(chfft_plan plan, const void* in, void* out, int method, int
     const Launch& l0 = L[li];
     const void* slots_src[3] = {in, out, plan->scratch.p};
     void* slots_dst[3] = {const_cast<void*>(in), out, plan->scratch.p};
+    if (l0.src_off) slots_src[l0.src] = static_cast<const char*>(slots_src[l0.src]) + l0.src_off;
+    if (l0.dst_off) slots_dst[l0.dst] = static_cast<char*>(slots_dst[l0.dst]) + l0.dst_off;
     if (l0.kind == 2) {
       if (launch_fourstep(plan->f64, l0.nph, reinterpret_cast<const KParams*>(l0.qdev), slots_src[l0.src],
                           slots_dst[l0.dst], scale * l0.scale_mul, l0.batch, l0.src_dist, l0.dst_dist,

@@ -266,13 +266,18 @@ def test_fourstep_schedules(monkeypatch, schedule, n, batch, dtype):
         check(got, oracle.c2c(xr, m), f64, n)
 
 
+@pytest.mark.parametrize("chunk", [None, "200", "-1"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_headline_full_batch(dtype):
-    # N = 43200 at the full batch of 512 on device buffers: column workgroups
-    # then walk runs of several transforms; spot-check transforms across the
-    # batch (first, both ends of interior runs, last) against the oracle
+def test_headline_full_batch(dtype, chunk, monkeypatch):
+    # N = 43200 at the full batch of 512 on device buffers, split into the
+    # planner's chunks (FP32: 2 x 256, FP64: 4 x 128), uneven chunks of
+    # 200 + 200 + 112 and one launch pair over the whole batch; spot-check
+    # transforms across the batch (first, both ends of chunks, last) against
+    # the oracle
     import torch
     from chfft_amd import Plan
+    if chunk is not None:
+        monkeypatch.setenv("CHFFT_FS_CHUNK", chunk)
     f64 = dtype == np.float64
     cdt = np.complex128 if f64 else np.complex64
     n, batch = 43200, 512
@@ -285,7 +290,7 @@ def test_headline_full_batch(dtype):
         p.exec(xd, yd, m)
         torch.cuda.synchronize()
         y = yd.cpu().numpy()
-        for b in (0, 1, 7, 8, 255, 256, 510, 511):
+        for b in (0, 1, 127, 128, 199, 200, 255, 256, 399, 400, 511):
             check(y[b], oracle.c2c(x[b].astype(np.complex128), m), f64, n)
 
 
