@@ -1232,6 +1232,7 @@ int chfft_plan_c2c_2d(chfft_plan* plan, size_t len_m, size_t len_n, size_t batch
       Geometry g = choose_geometry((int)p->inner.Lfft, c.P.G, true, p->f64);
       C2CEngine::finish(c, g, S_OUT, S_OUT, 1);
       c.what = "2-D columns " + seq_str(p->inner.seq);
+      try_specialise(c, p->f64);
       L.push_back(c);
     } else {
       // a single row: undo the conjugation and scale with a plain pass
@@ -1240,8 +1241,9 @@ int chfft_plan_c2c_2d(chfft_plan* plan, size_t len_m, size_t len_n, size_t batch
     }
   }
   // row pass of an inverse: conjugate on load only, keep conjugated output
-  for (auto& l : p->bwd)
-    if (l.what.rfind("2-D columns", 0) != 0 && len_m > 1) l.P.conj_out = 0;
+  // (every launch but the last, the column pass)
+  if (len_m > 1)
+    for (size_t i = 0; i + 1 < p->bwd.size(); i++) p->bwd[i].P.conj_out = 0;
   return finish_plan(p, plan);
 }
 

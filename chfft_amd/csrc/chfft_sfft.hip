@@ -1051,6 +1051,9 @@ static const SfftEntry kSfft[] = {
     SF_BOTH(512, 32, 8, 4, false, LD_IMDCT, ST_IMDCT, 0, false, 3, 16, 8, 4),
     // 2048: CFft2D rows, CFft1D
     SF_BOTH(2048, 128, 2, 1, false, LD_PLAIN, ST_PLAIN, 0, false, 3, 16, 16, 8),
+    // 2048 columns (CFft2D 2048 x 2048 second axis): 64-byte row pieces of
+    // 8 (FP32) / 4 (FP64) columns, one 128 KB tile per CU
+    SF_BOTH(2048, 128, 8, 4, true, LD_PLAIN, ST_PLAIN, 0, false, 3, 16, 16, 8),
     // 8192: fused Bluestein for N <= 4096 (N = 4093 and smaller primes), CFft1D
     SF_BOTH(8192, 512, 1, 1, false, LD_PLAIN, ST_PLAIN, 1, false, 4, 16, 16, 8, 4),
     SF_BOTH(8192, 512, 1, 1, false, LD_PLAIN, ST_PLAIN, 0, false, 4, 16, 16, 8, 4),
