@@ -54,6 +54,7 @@ struct SfftEntry {
   int nrad;
   int rad[8];
   void (*fn)(KParams);
+  void (*fn0)(KParams);
 };
 const SfftEntry* sfft_find(bool f64, int L, int ld, int st, int mode, bool col);
 bool sfft_has_schedule(bool f64, int L, const int* rad, int nrad);

@@ -489,7 +489,19 @@ __device__ __forceinline__ void gather_generic(const KParams& P, const void* src
 
 // First-pass gather of one tile: element i + j*L/R0 of butterfly i = u + b*TPT
 // of transform g0 + t.
-template <typename T, int L, int TPT, int TPB, bool COL, int LD, int MODE, class RS>
+// Paired-real fast paths need the real buffer 8-byte (FP32) / 16-byte (FP64)
+// aligned with an even distance between transforms. AL selects how a kernel
+// decides: 2 tests at run time, 1 takes the fast paths (the kernel has
+// tested every such condition once, up front), 0 takes the per-element
+// paths. Separate AL = 1 and AL = 0 bodies keep each one's register arrays
+// apart (one body holding both paths kept some of them in scratch memory).
+template <int AL>
+__host__ __device__ __forceinline__ bool pair_aligned(const void* p, long long bdist, size_t esz) {
+  if constexpr (AL == 2) return (reinterpret_cast<uintptr_t>(p) & (esz - 1)) == 0 && (bdist & 1) == 0;
+  return AL == 1;
+}
+
+template <typename T, int L, int TPT, int TPB, bool COL, int LD, int MODE, class RS, int AL>
 __device__ __forceinline__ void sfft_gather(const KParams& P, const void* src, int g0,
                                             cx<T> (&x)[Plan<RS, L, TPT>::EMAX()]) {
   using PL = Plan<RS, L, TPT>;
@@ -569,7 +581,7 @@ __device__ __forceinline__ void sfft_gather(const KParams& P, const void* src, i
     // (two contiguous complex-sized loads per slot), keeps element n and hands
     // element L-1-n -- slot R0-1-j of lane TPT-1-u, the mirror lane of the
     // same transform -- across with one lane shuffle per real.
-    if ((reinterpret_cast<uintptr_t>(src) & (sizeof(cx<T>) - 1)) == 0 && (P.in.bdist & 1) == 0) {  // uniform
+    if (pair_aligned<AL>(src, P.in.bdist, sizeof(cx<T>))) {  // uniform
       const T* xr = reinterpret_cast<const T*>(src) + (long long)(valid ? g : g0) * P.in.bdist;
       const T o = (T)P.ortho;
       const bool cji = P.conj_in != 0;
@@ -642,7 +654,7 @@ __device__ __forceinline__ void sfft_gather(const KParams& P, const void* src, i
     // four aligned pairs of reals (and of window values) serve both. Lane u
     // folds both elements of its slots j < R0/2 and hands element L-1-n to
     // the mirror lane TPT-1-u (slot R0-1-j) with one shuffle per real.
-    if ((reinterpret_cast<uintptr_t>(src) & (sizeof(cx<T>) - 1)) == 0 && (P.in.bdist & 1) == 0) {  // uniform
+    if (pair_aligned<AL>(src, P.in.bdist, sizeof(cx<T>))) {  // uniform
       const T* xr = reinterpret_cast<const T*>(src) + (long long)(valid ? g : g0) * P.in.bdist;
       const T* wr = reinterpret_cast<const T*>(P.tab_b);
       const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
@@ -706,7 +718,7 @@ struct StoreTw {
 // MODE: 0 plain passes; 1 fused Bluestein (chirp on load, spectrum product,
 // second pass sequence, de-chirp); 2 spectrum product + second sequence only
 // (the row pass of a four-step Bluestein).
-template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS>
+template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS, int AL>
 __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, void* dst, int g0, double scale,
                                           cx<T>* lds, const StoreTw<T, Plan<RS, L, TPT>, TPT, ST, EPI>& stw) {
   constexpr bool BLUE = MODE == 1;
@@ -739,7 +751,7 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
 
   constexpr int RL = PL::R(NP - 1), PLAST = PL::P(NP - 1), BL = PL::B(NP - 1);
   cx<T> x[E];
-  sfft_gather<T, L, TPT, TPB, COL, LD, MODE, RS>(P, src, g0, x);
+  sfft_gather<T, L, TPT, TPB, COL, LD, MODE, RS, AL>(P, src, g0, x);
   // per-pass twiddle tables copied into LDS behind the tile buffers
   const cx<T>* twp = tw;
   if constexpr (LDT) {
@@ -813,7 +825,7 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
     } else if constexpr (ST == ST_DCT2 && L % 2 == 0) {
       epi_dct2<T, L, TPT, TPB>(P, dst, img, LP, g0, nt, (T)scale);
     } else if constexpr (ST == ST_DCT3 && L % 2 == 0) {
-      if ((reinterpret_cast<uintptr_t>(dst) & (sizeof(cx<T>) - 1)) == 0 && (P.out.bdist & 1) == 0) {
+      if (pair_aligned<AL>(dst, P.out.bdist, sizeof(cx<T>))) {
         epi_dct3<T, L, TPT, TPB>(P, dst, img, LP, g0, nt, (T)scale);
       } else {
         const TMap m = make_map(TPB);
@@ -862,7 +874,7 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
       T* ret = reinterpret_cast<T*>(dst) + (long long)g * P.out.bdist;
       constexpr unsigned RE = sizeof(T), CE = sizeof(cx<T>);
       const bool cjo = P.conj_out != 0;
-      if ((reinterpret_cast<uintptr_t>(dst) & (sizeof(cx<T>) - 1)) == 0 && (P.out.bdist & 1) == 0) {  // uniform
+      if (pair_aligned<AL>(dst, P.out.bdist, sizeof(cx<T>))) {  // uniform
         T wim[BL * RL], wre[BL * RL];
 #pragma unroll
         for (int b = 0; b < BL; b++)
@@ -919,7 +931,7 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
   }
 }
 
-template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS>
+template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS, int AL = 2>
 __device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, void* dst, int g0, double scale,
                                           cx<T>* lds) {
   StoreTw<T, Plan<RS, L, TPT>, TPT, ST, EPI> stw;
@@ -931,7 +943,7 @@ __device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, voi
     io_base(P.out, min(g0 + t, P.G - 1), &c);
     stw.load(P, u, c);
   }
-  sfft_body<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, RS>(P, src, dst, g0, scale, lds, stw);
+  sfft_body<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, RS, AL>(P, src, dst, g0, scale, lds, stw);
 }
 
 // Tile index of this workgroup, arranged so that the workgroups one XCD runs
@@ -950,12 +962,12 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return first + b / X;
 }
 
-template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS>
+template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS, int AL>
 __global__ void __launch_bounds__(TPT * TPB) sfft_kernel(KParams P) {
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
   const int tile = (COL || P.colmajor_st) ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-  sfft_tile<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, RS>(P, P.src, P.dst, tile * TPB, P.scale, lds);
+  sfft_tile<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, RS, AL>(P, P.src, P.dst, tile * TPB, P.scale, lds);
 }
 
 // Persistent four-step transform with length-specialised phases: one
@@ -1024,15 +1036,27 @@ struct SfftEntry {
   int f64, L, tpt, tpb, col, ld, st, mode, epi;
   int nrad;
   int rad[8];
-  void (*fn)(KParams);
+  void (*fn)(KParams);   // the kernel (paired-real operators: aligned buffers)
+  void (*fn0)(KParams);  // paired-real operators on unaligned buffers (else fn)
 };
 
-#define SF_KERNEL(T, L, TPT, TPB, COL, LD, ST, MODE, EPI, ...) \
-  &sfft_kernel<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, Rad<__VA_ARGS__>>
+// Operators with paired-real fast paths get two kernels: one that assumes the
+// alignment those paths need and one with per-element accesses only; the
+// launch picks one from the buffers (see pair_aligned).
+constexpr bool sf_paired(int ld, int st) {
+  return ld == LD_DCT2 || ld == LD_MDCT || st == ST_DCT3 || st == ST_MDCT;
+}
+constexpr int sf_al(int ld, int st, bool aligned) { return sf_paired(ld, st) ? (aligned ? 1 : 0) : 2; }
 
-#define SF_ENTRY(F64, T, L, TPT, TPB, COL, LD, ST, MODE, EPI, NR, ...)                   \
-  {F64, L, TPT, TPB, COL, LD, ST, MODE, EPI, NR, {__VA_ARGS__},                          \
-   reinterpret_cast<void (*)(KParams)>(SF_KERNEL(T, L, TPT, TPB, COL, LD, ST, MODE, EPI, __VA_ARGS__))}
+#define SF_KERNEL(T, L, TPT, TPB, COL, LD, ST, MODE, EPI, AL, ...) \
+  &sfft_kernel<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, Rad<__VA_ARGS__>, AL>
+
+#define SF_ENTRY(F64, T, L, TPT, TPB, COL, LD, ST, MODE, EPI, NR, ...)                                      \
+  {F64, L, TPT, TPB, COL, LD, ST, MODE, EPI, NR, {__VA_ARGS__},                                             \
+   reinterpret_cast<void (*)(KParams)>(SF_KERNEL(T, L, TPT, TPB, COL, LD, ST, MODE, EPI, sf_al(LD, ST, true), \
+                                                 __VA_ARGS__)),                                             \
+   reinterpret_cast<void (*)(KParams)>(SF_KERNEL(T, L, TPT, TPB, COL, LD, ST, MODE, EPI, sf_al(LD, ST, false), \
+                                                 __VA_ARGS__))}
 
 #define SF_BOTH(L, TPT, TPB, TPB64, COL, LD, ST, MODE, EPI, NR, ...)                    \
   SF_ENTRY(0, float, L, TPT, TPB, COL, LD, ST, MODE, EPI, NR, __VA_ARGS__),              \
@@ -1159,7 +1183,14 @@ bool sfft_has_schedule(bool f64, int L, const int* rad, int nrad) {
 }
 
 hipError_t sfft_launch(const SfftEntry* e, const KParams& P, dim3 grid, size_t lds, hipStream_t st) {
-  hipLaunchKernelGGL(e->fn, grid, dim3(e->tpt * e->tpb), lds, st, P);
+  void (*fn)(KParams) = e->fn;
+  if (e->fn0 != e->fn) {
+    const size_t esz = e->f64 ? 16 : 8;
+    const bool al = (!(e->ld == LD_DCT2 || e->ld == LD_MDCT) || pair_aligned<2>(P.src, P.in.bdist, esz)) &&
+                    (!(e->st == ST_DCT3 || e->st == ST_MDCT) || pair_aligned<2>(P.dst, P.out.bdist, esz));
+    if (!al) fn = e->fn0;
+  }
+  hipLaunchKernelGGL(fn, grid, dim3(e->tpt * e->tpb), lds, st, P);
   return hipGetLastError();
 }
 
@@ -1169,11 +1200,12 @@ hipError_t sfft_set_lds_limits() {
     hipError_t x = hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes);
     if (x != hipSuccess) r = x;
   }
-  for (const SfftEntry& e : kSfft) {
-    hipError_t x = hipFuncSetAttribute(reinterpret_cast<const void*>(e.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)kLdsBytes);
-    if (x != hipSuccess) r = x;
-  }
+  for (const SfftEntry& e : kSfft)
+    for (void (*fn)(KParams) : {e.fn, e.fn0}) {
+      hipError_t x = hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)kLdsBytes);
+      if (x != hipSuccess) r = x;
+    }
   return r;
 }
 
