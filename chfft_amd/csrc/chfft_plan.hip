@@ -47,8 +47,8 @@ hipError_t launch_multi(bool f64, const MultiItem* items, const MultiUnit* units
 int multi_occupancy(bool f64, size_t lds);
 hipError_t gen_roots(bool f64, const RootJob* jobs_dev, int njobs, void* arena, hipStream_t st);
 hipError_t gen_rot(const long long* ns_dev, int count, long long M, void* out, hipStream_t st);
-hipError_t put_spectra(bool f64, const void* src, int count, long long M, const long long* offs_dev, void* arena,
-                       hipStream_t st);
+hipError_t put_spectra(bool f64, const void* src, int count, long long M, long long n1, const long long* offs_dev,
+                       void* arena, hipStream_t st);
 struct SfftEntry {
   int f64, L, tpt, tpb, col, ld, st, mode, epi;
   int nrad;
@@ -1864,7 +1864,9 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
         chfft_plan sub = nullptr;
         if ((rc = chfft_plan_c2c_1d(&sub, M, cnt, CHFFT_F64))) return rc;
         rc = chfft_exec(sub, buf.p, buf.p, CHFFT_FORWARD, nullptr);
-        if (!rc && put_spectra(f64, buf.p, (int)cnt, (long long)M, reinterpret_cast<const long long*>(od.p),
+        const LenInfo& L0 = info[ns[b0]];  // every length of this M has the same split
+        if (!rc && put_spectra(f64, buf.p, (int)cnt, (long long)M, L0.kind == 2 ? (long long)L0.n1 : 0,
+                               reinterpret_cast<const long long*>(od.p),
                                p->arena.p, nullptr) != hipSuccess)
           rc = CHFFT_ERR_HIP;
         if (!rc && hipDeviceSynchronize() != hipSuccess) rc = CHFFT_ERR_HIP;
@@ -1938,9 +1940,9 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
           A.chirp = at(L.chirp);
           B.out = B.in;
           B.has_mid = 1;
-          B.kspec = at(L.kspec);
-          B.kc = 1;
-          B.ke = (long long)L.n1;
+          B.kspec = at(L.kspec);  // row-major [k1][k2]
+          B.kc = (long long)L.n2;
+          B.ke = 1;
           B.twsh = L.twsh;
           B.tab_a = at(L.tw_hi);
           B.tab_b = at(L.tw_lo);

@@ -59,14 +59,18 @@ __global__ void gen_rot_kernel(const long long* __restrict__ ns, int count, long
 // dst[i] = (T)(src[i] / M) for the spectra of `count` lengths; spectrum b goes
 // to arena offset offs[b].
 template <typename T>
-__global__ void put_spectra_kernel(const cx<double>* __restrict__ src, int count, long long M,
+__global__ void put_spectra_kernel(const cx<double>* __restrict__ src, int count, long long M, long long n1,
                                    const long long* __restrict__ offs, cx<T>* __restrict__ arena) {
   const long long total = (long long)count * M;
   const double inv = 1.0 / (double)M;
+  const long long n2 = n1 > 0 ? M / n1 : 0;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (long long)gridDim.x * blockDim.x) {
     const long long b = e / M, i = e - b * M;
-    const cx<double> v = src[e];
+    // four-step items store bin k1 + n1 * k2 at k1 * n2 + k2 (row-major for
+    // the row pass, which reads row k1's bins with consecutive lanes)
+    const long long k = n1 > 0 ? (i / n2) + n1 * (i % n2) : i;
+    const cx<double> v = src[b * M + k];
     arena[offs[b] + i] = mk((T)(v.re * inv), (T)(v.im * inv));
   }
 }
@@ -89,15 +93,15 @@ hipError_t gen_rot(const long long* ns_dev, int count, long long M, void* out, h
   return hipGetLastError();
 }
 
-hipError_t put_spectra(bool f64, const void* src, int count, long long M, const long long* offs_dev, void* arena,
-                       hipStream_t st) {
+hipError_t put_spectra(bool f64, const void* src, int count, long long M, long long n1, const long long* offs_dev,
+                       void* arena, hipStream_t st) {
   if (f64)
     hipLaunchKernelGGL(put_spectra_kernel<double>, dim3(1024), dim3(256), 0, st,
-                       reinterpret_cast<const cx<double>*>(src), count, M, offs_dev,
+                       reinterpret_cast<const cx<double>*>(src), count, M, n1, offs_dev,
                        reinterpret_cast<cx<double>*>(arena));
   else
     hipLaunchKernelGGL(put_spectra_kernel<float>, dim3(1024), dim3(256), 0, st,
-                       reinterpret_cast<const cx<double>*>(src), count, M, offs_dev,
+                       reinterpret_cast<const cx<double>*>(src), count, M, n1, offs_dev,
                        reinterpret_cast<cx<float>*>(arena));
   return hipGetLastError();
 }
