@@ -1089,7 +1089,7 @@ static const SfftEntry kSfft[] = {
     // inverse columns + de-chirp; M = 20480 = 128 x 160 (N = 10007),
     // M = 32768 = 128 x 256, M = 147456 = 384 x 384 (N = 65537)
     SF_BOTH(128, 16, 16, 16, true, LD_FS_CHIRP, ST_TW, 0, false, 2, 16, 8),
-    SF_BOTH(160, 16, 4, 4, false, LD_PLAIN, ST_TW, 2, false, 3, 8, 4, 5),
+    SF_BOTH(160, 16, 8, 4, false, LD_PLAIN, ST_TW, 2, false, 3, 8, 4, 5),
     SF_BOTH(256, 16, 4, 4, false, LD_PLAIN, ST_TW, 2, false, 2, 16, 16),
     SF_BOTH(128, 16, 16, 16, true, LD_PLAIN, ST_FS_DECHIRP, 0, false, 2, 16, 8),
     SF_BOTH(384, 32, 16, 8, true, LD_FS_CHIRP, ST_TW, 0, false, 3, 16, 8, 3),

@@ -249,6 +249,29 @@ def test_cfft2d_config_size(dtype):
     check(f.backward(y), x.astype(np.complex128), f64, m * n)
 
 
+@pytest.mark.parametrize("f64", [False, True])
+def test_benchmark_schedules(f64):
+    # the benchmark shapes run on the length-specialised kernels with the
+    # schedules they were tuned with: 43200 = 180 x 240 as prime-factor
+    # passes (12 x 15, 16 x 15) in equal chunks whose scratch stays below
+    # 96 MB, the 2-D 2048-point columns on their own tile kernel
+    from chfft_amd import Plan
+    d = Plan("c2c", f64, 512, 43200).describe()
+    fwd = d.split("inverse:")[0].strip().splitlines()[1:]
+    chunks, per = (4, 128) if f64 else (2, 256)
+    assert len(fwd) == 2 * chunks, d
+    cols = "specialised four-step columns L=180 radix[12,15] + twiddle (transforms %d+)"
+    rows = "specialised four-step rows L=240 radix[16,15] + transpose (transforms %d+)"
+    for c in range(chunks):
+        assert fwd[2 * c].strip().startswith(cols % (c * per)), fwd[2 * c]
+        assert fwd[2 * c + 1].strip().startswith(rows % (c * per)), fwd[2 * c + 1]
+    d2 = Plan("c2c2d", f64, 1, 2048, 2048).describe()
+    assert "specialised 2-D columns L=2048" in d2, d2
+    for n, b in ((10007, 512), (65537, 64)):
+        lines = Plan("c2c", f64, b, n).describe().split("inverse:")[0].strip().splitlines()[1:]
+        assert len(lines) == 3 and all(l.strip().startswith("specialised bluestein") for l in lines), lines
+
+
 @pytest.mark.parametrize("schedule", ["persistent", "split"])
 @pytest.mark.parametrize("n,batch", [(43200, 96), (10007, 64), (20000, 40), (3 ** 9, 33)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
