@@ -487,6 +487,17 @@ __device__ __forceinline__ void gather_generic(const KParams& P, const void* src
   }
 }
 
+// Fused Bluestein pads N points to M >= 2N, so the upper half of the first
+// sequence is zero and only outputs below M / 2 are kept. In a pass of even
+// radix R over L points, slot j of a butterfly (element i + j L / R, i < L / R)
+// lies in the lower half exactly when 2 j < R: a compile-time test, so the
+// compiler drops the zero inputs' arithmetic in the first pass and the unused
+// outputs' in the last.
+template <int R>
+__device__ __forceinline__ constexpr bool lower_half(int j) {
+  return R % 2 != 0 || 2 * j < R;
+}
+
 // First-pass gather of one tile: element i + j*L/R0 of butterfly i = u + b*TPT
 // of transform g0 + t.
 // Paired-real fast paths need the real buffer 8-byte (FP32) / 16-byte (FP64)
@@ -532,7 +543,7 @@ __device__ __forceinline__ void sfft_gather(const KParams& P, const void* src, i
         for (int j = 0; j < R0; j++) {
           const int rel = b * TPT + j * NB0;  // element n = u + rel
           cx<T> v = mk(T(0), T(0));
-          if (!BLUE || u + rel < P.L) {
+          if (!BLUE || (lower_half<R0>(j) && u + rel < P.L)) {
             v = ld_off<cx<T>>(tb + (unsigned)rel * es, voff);
             if (cji) v = cj(v);
             if constexpr (BLUE) v = v * ld_off<cx<T>>(chirp + rel, (unsigned)u * ESZ);
@@ -797,7 +808,7 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
 #pragma unroll
         for (int j = 0; j < RL; j++) {
           const int rel = b * TPT + j * PLAST;  // k = u + rel
-          if (u + rel < P.L) x[b * RL + j] = cj(x[b * RL + j]) * ld_off<cx<T>>(chirp + rel, (unsigned)u * ESZ);
+          if (lower_half<RL>(j) && u + rel < P.L) x[b * RL + j] = cj(x[b * RL + j]) * ld_off<cx<T>>(chirp + rel, (unsigned)u * ESZ);
         }
       }
     }
@@ -851,7 +862,7 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
 #pragma unroll
           for (int j = 0; j < RL; j++) {
             const int rel = b * TPT + j * PLAST;  // k = u + rel
-            if (!BLUE || u + rel < P.L) {
+            if (!BLUE || (lower_half<RL>(j) && u + rel < P.L)) {
               cx<T> v = x[b * RL + j];
               if constexpr (TW_EARLY)
                 v = v * stw.w[b * RL + j];
@@ -923,7 +934,7 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
 #pragma unroll
           for (int j = 0; j < RL; j++) {
             const int k = i + j * PLAST;
-            if (!BLUE || k < P.L) store_elem<T, ST>(P, dst, obase, c, k, x[b * RL + j], s);
+            if (!BLUE || (lower_half<RL>(j) && k < P.L)) store_elem<T, ST>(P, dst, obase, c, k, x[b * RL + j], s);
           }
         }
       }
