@@ -493,17 +493,24 @@ size_t fourstep_split(size_t total, bool f64) {
 // linear convolution of the chirped input with the chirp, so the results
 // agree to rounding. Taken here: the 5-smooth M in [2N - 1, 2 * next_pow2(N)]
 // with an aligned split (n2 % 16 == 0, n2 fusable) that needs the least work,
-// M * (passes of n1 + passes of n2) -- 20480 = 128 x 160 for N = 10007 and
-// 147456 = 384 x 384 for N = 65537 instead of 32768 and 262144.
+// M * (passes of n1 + passes of n2), a factor counting its prime-factor
+// schedule where a specialised kernel runs it -- 20480 = 128 x 160 for
+// N = 10007 and, for N = 65537, 138240 = 360 x 384 (360 = 8 x 15 x 3, FP32)
+// or 147456 = 384 x 384 (FP64), instead of 32768 and 262144.
 size_t bluestein_length(size_t N, bool f64) {
   const size_t hi = 2 * next_pow2(N);
   size_t best = hi;
   double best_cost = 1e300;
+  auto passes = [&](size_t L) {
+    const std::vector<int> rp = radices_for(L, true);
+    const size_t plain = radices_for(L).size();
+    return sfft_has_schedule(f64, (int)L, rp.data(), (int)rp.size()) ? std::min(plain, rp.size()) : plain;
+  };
   for (size_t m = 2 * N - 1; m <= hi; m++) {
     if (!smooth5(m)) continue;
     const size_t d = fourstep_split(m, f64);
     if (d == 0 || (m / d) % 16 != 0 || (m / d) * 4 > lds_cap(f64)) continue;
-    const double cost = (double)m * (double)(radices_for(d).size() + radices_for(m / d).size());
+    const double cost = (double)m * (double)(passes(d) + passes(m / d));
     if (cost < best_cost) {
       best_cost = cost;
       best = m;
