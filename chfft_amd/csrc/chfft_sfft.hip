@@ -94,9 +94,16 @@ constexpr int lds_stride(int L, int tpb) {
 }
 // LDS bytes of one tile: one padded buffer, or two (ping-pong) when both
 // fit in 16 KB, plus the per-pass twiddle tables when they are kept in LDS.
-constexpr size_t lds_need(int L, int tpb, int esz, int ntw) {
+// LDS (complex elements) of the MDCT fold's shared tables: the window (4L
+// reals, as 2L pairs) and the L fold twiddles, read by every transform of a
+// tile (mdct1d.rs:206-240). FP32 only: measured on MI355X (2048-point MDCT
+// over 2^24 samples) FP32 50.5 -> 41.3 us, FP64 70 -> 75 us (its tables
+// are twice the LDS), so FP64 keeps reading them from the caches.
+constexpr int mdct_tab_elems(int L, int ld, int esz) { return ld == LD_MDCT && esz == 8 ? 3 * L : 0; }
+
+constexpr size_t lds_need(int L, int tpb, int esz, int ntw, int extra = 0) {
   const size_t one = (size_t)tpb * lds_stride(L, tpb) * esz;
-  return (2 * one <= 16384 ? 2 * one : one) + (tw_in_lds(ntw, esz) ? (size_t)ntw * esz : 0);
+  return (2 * one <= 16384 ? 2 * one : one) + (tw_in_lds(ntw, esz) ? (size_t)ntw * esz : 0) + (size_t)extra * esz;
 }
 
 // Workgroup barrier for LDS traffic only: waits for this wave's LDS accesses
@@ -514,7 +521,7 @@ __host__ __device__ __forceinline__ bool pair_aligned(const void* p, long long b
 
 template <typename T, int L, int TPT, int TPB, bool COL, int LD, int MODE, class RS, int AL>
 __device__ __forceinline__ void sfft_gather(const KParams& P, const void* src, int g0,
-                                            cx<T> (&x)[Plan<RS, L, TPT>::EMAX()]) {
+                                            cx<T> (&x)[Plan<RS, L, TPT>::EMAX()], cx<T>* mtab) {
   using PL = Plan<RS, L, TPT>;
   constexpr bool BLUE = MODE == 1;
   constexpr unsigned ESZ = sizeof(cx<T>);
@@ -665,31 +672,73 @@ __device__ __forceinline__ void sfft_gather(const KParams& P, const void* src, i
     // four aligned pairs of reals (and of window values) serve both. Lane u
     // folds both elements of its slots j < R0/2 and hands element L-1-n to
     // the mirror lane TPT-1-u (slot R0-1-j) with one shuffle per real.
+    // The window and the fold twiddles are the same for every transform of
+    // the tile: in FP32 the workgroup copies them into LDS (mtab) once, after
+    // its signal loads are in flight, and every lane reads them from there.
     if (pair_aligned<AL>(src, P.in.bdist, sizeof(cx<T>))) {  // uniform
       const T* xr = reinterpret_cast<const T*>(src) + (long long)(valid ? g : g0) * P.in.bdist;
-      const T* wr = reinterpret_cast<const T*>(P.tab_b);
+      const cx<T>* wr = reinterpret_cast<const cx<T>*>(P.tab_b);  // window pairs
       const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
       constexpr int q = L, q3 = 3 * L;
       constexpr unsigned RE = sizeof(T), CE = sizeof(cx<T>);
       const bool cji = P.conj_in != 0;
+      if constexpr (mdct_tab_elems(L, LD, (int)sizeof(cx<T>)) > 0) {
+        cx<T> pr[R0 / 2][4];
 #pragma unroll
-      for (int j = 0; j < R0 / 2; j++) {
-        const int n = u + j * TPT, m = 2 * n;
-        const cx<T> p1 = ld_off<cx<T>>(xr, (unsigned)(q + m) * RE);           // (A3, B1)
-        const cx<T> p2 = ld_off<cx<T>>(xr, (unsigned)(q - 2 - m) * RE);       // (B2, A4)
-        const cx<T> p3 = ld_off<cx<T>>(xr, (unsigned)(q3 - 2 - m) * RE);      // (B3, A1)
-        const cx<T> p4 = ld_off<cx<T>>(xr, (unsigned)(q3 + m) * RE);          // (A2, B4)
-        const cx<T> w1 = ld_off<cx<T>>(wr, (unsigned)(q + m) * RE);
-        const cx<T> w2 = ld_off<cx<T>>(wr, (unsigned)(q - 2 - m) * RE);
-        const cx<T> w3 = ld_off<cx<T>>(wr, (unsigned)(q3 - 2 - m) * RE);
-        const cx<T> w4 = ld_off<cx<T>>(wr, (unsigned)(q3 + m) * RE);
-        cx<T> va = mk(p3.im * w3.im + p4.re * w4.re, p1.re * w1.re - p2.im * w2.im);
-        cx<T> vb = mk(p1.im * w1.im - p2.re * w2.re, p3.re * w3.re + p4.im * w4.im);
-        va = va * ld_off<cx<T>>(tw, (unsigned)n * CE);
-        vb = vb * ld_off<cx<T>>(tw, (unsigned)(L - 1 - n) * CE);
-        const cx<T> other = mk(__shfl_xor(vb.re, TPT - 1), __shfl_xor(vb.im, TPT - 1));
-        x[j] = cji ? cj(va) : va;
-        x[R0 - 1 - j] = cji ? cj(other) : other;
+        for (int j = 0; j < R0 / 2; j++) {
+          const int m = 2 * (u + j * TPT);
+          pr[j][0] = ld_off<cx<T>>(xr, (unsigned)(q + m) * RE);       // (A3, B1)
+          pr[j][1] = ld_off<cx<T>>(xr, (unsigned)(q - 2 - m) * RE);   // (B2, A4)
+          pr[j][2] = ld_off<cx<T>>(xr, (unsigned)(q3 - 2 - m) * RE);  // (B3, A1)
+          pr[j][3] = ld_off<cx<T>>(xr, (unsigned)(q3 + m) * RE);      // (A2, B4)
+        }
+        constexpr int NT = 3 * L, TH = TPT * TPB;
+        const int tid = threadIdx.x;
+#pragma unroll
+        for (int e0 = 0; e0 < NT; e0 += TH)
+          if (e0 + tid < NT)
+            mtab[e0 + tid] = e0 + tid < 2 * L ? ld_off<cx<T>>(wr, (unsigned)(e0 + tid) * CE)
+                                              : ld_off<cx<T>>(tw, (unsigned)(e0 + tid - 2 * L) * CE);
+        lds_barrier();
+        const cx<T>* wl = mtab;          // window pair i holds reals 2i, 2i + 1
+        const cx<T>* tl = mtab + 2 * L;  // fold twiddles
+#pragma unroll
+        for (int j = 0; j < R0 / 2; j++) {
+          const int n = u + j * TPT, m = 2 * n;
+          const cx<T> p1 = pr[j][0], p2 = pr[j][1], p3 = pr[j][2], p4 = pr[j][3];
+          const cx<T> w1 = wl[(q + m) / 2];
+          const cx<T> w2 = wl[(q - 2 - m) / 2];
+          const cx<T> w3 = wl[(q3 - 2 - m) / 2];
+          const cx<T> w4 = wl[(q3 + m) / 2];
+          cx<T> va = mk(p3.im * w3.im + p4.re * w4.re, p1.re * w1.re - p2.im * w2.im);
+          cx<T> vb = mk(p1.im * w1.im - p2.re * w2.re, p3.re * w3.re + p4.im * w4.im);
+          va = va * tl[n];
+          vb = vb * tl[L - 1 - n];
+          const cx<T> other = mk(__shfl_xor(vb.re, TPT - 1), __shfl_xor(vb.im, TPT - 1));
+          x[j] = cji ? cj(va) : va;
+          x[R0 - 1 - j] = cji ? cj(other) : other;
+        }
+      } else {
+        (void)mtab;
+#pragma unroll
+        for (int j = 0; j < R0 / 2; j++) {
+          const int n = u + j * TPT, m = 2 * n;
+          const cx<T> p1 = ld_off<cx<T>>(xr, (unsigned)(q + m) * RE);       // (A3, B1)
+          const cx<T> p2 = ld_off<cx<T>>(xr, (unsigned)(q - 2 - m) * RE);   // (B2, A4)
+          const cx<T> p3 = ld_off<cx<T>>(xr, (unsigned)(q3 - 2 - m) * RE);  // (B3, A1)
+          const cx<T> p4 = ld_off<cx<T>>(xr, (unsigned)(q3 + m) * RE);      // (A2, B4)
+          const cx<T> w1 = ld_off<cx<T>>(wr, (unsigned)(q + m) * RE);
+          const cx<T> w2 = ld_off<cx<T>>(wr, (unsigned)(q - 2 - m) * RE);
+          const cx<T> w3 = ld_off<cx<T>>(wr, (unsigned)(q3 - 2 - m) * RE);
+          const cx<T> w4 = ld_off<cx<T>>(wr, (unsigned)(q3 + m) * RE);
+          cx<T> va = mk(p3.im * w3.im + p4.re * w4.re, p1.re * w1.re - p2.im * w2.im);
+          cx<T> vb = mk(p1.im * w1.im - p2.re * w2.re, p3.re * w3.re + p4.im * w4.im);
+          va = va * ld_off<cx<T>>(tw, (unsigned)n * CE);
+          vb = vb * ld_off<cx<T>>(tw, (unsigned)(L - 1 - n) * CE);
+          const cx<T> other = mk(__shfl_xor(vb.re, TPT - 1), __shfl_xor(vb.im, TPT - 1));
+          x[j] = cji ? cj(va) : va;
+          x[R0 - 1 - j] = cji ? cj(other) : other;
+        }
       }
     } else {
       gather_generic<T, L, TPT, TPB, COL, LD, MODE, RS>(P, src, g0, x);
@@ -744,7 +793,8 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
   constexpr int NTW = rad_twoff(RS{}, NP);          // full tables
   constexpr int NTWC = rad_compact_off(RS{}, NP);   // compact tables, after the full ones
   constexpr bool LDT = tw_in_lds(NTWC, (int)sizeof(cx<T>));
-  static_assert(((PP ? 2 : 1) * TPB * LP + (LDT ? NTWC : 0)) * sizeof(cx<T>) <= kLdsBytes,
+  constexpr int MTAB = mdct_tab_elems(L, LD, (int)sizeof(cx<T>));
+  static_assert(((PP ? 2 : 1) * TPB * LP + (LDT ? NTWC : 0) + MTAB) * sizeof(cx<T>) <= kLdsBytes,
                 "transform tile does not fit in LDS");
   const int tid = threadIdx.x;
   const int t = COL ? tid % TPB : tid / TPT;
@@ -762,7 +812,9 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
 
   constexpr int RL = PL::R(NP - 1), PLAST = PL::P(NP - 1), BL = PL::B(NP - 1);
   cx<T> x[E];
-  sfft_gather<T, L, TPT, TPB, COL, LD, MODE, RS, AL>(P, src, g0, x);
+  // the MDCT fold's tables, behind the tile buffers and the pass tables
+  cx<T>* mtab = lds + (PP ? 2 : 1) * TPB * LP + (LDT ? NTWC : 0);
+  sfft_gather<T, L, TPT, TPB, COL, LD, MODE, RS, AL>(P, src, g0, x, mtab);
   // per-pass twiddle tables copied into LDS behind the tile buffers
   const cx<T>* twp = tw;
   if constexpr (LDT) {
@@ -894,7 +946,11 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
             const int k = u + b * TPT + j * PLAST;
             cx<T> v = x[b * RL + j];
             if (cjo) v = cj(v);
-            const cx<T> w = v * scl(ld_off<cx<T>>(tw, (unsigned)k * CE), s);
+            // the aligned fold (AL == 1 with LD_MDCT, FP32) left the twiddles in LDS
+            const cx<T> t = (AL == 1 && mdct_tab_elems(L, LD, (int)sizeof(cx<T>)) > 0)
+                                ? mtab[2 * L + k]
+                                : ld_off<cx<T>>(tw, (unsigned)k * CE);
+            const cx<T> w = v * scl(t, s);
             wre[b * RL + j] = w.re;
             wim[b * RL + j] = w.im;
           }
@@ -1232,7 +1288,7 @@ size_t sfft_lds_bytes(const SfftEntry* e) {
     ntw += tw_compact_entries(e->rad[q], p);
     p *= e->rad[q];
   }
-  return lds_need(e->L, e->tpb, e->f64 ? 16 : 8, ntw);
+  return lds_need(e->L, e->tpb, e->f64 ? 16 : 8, ntw, mdct_tab_elems(e->L, e->ld, e->f64 ? 16 : 8));
 }
 
 int sfft_lds_stride(const SfftEntry* e) { return lds_stride(e->L, e->tpb); }
