@@ -4,7 +4,7 @@
  * Drop-in boundary for chfft 0.3.4's five public transform types
  * (lib.rs:21-25: CFft1D, CFft2D, Dct1D/DctType, Mdct1D, RFft1D). The reference
  * has no FFI of its own; each entry point below replaces the Rust method it
- * cites, and a language binding (the Rust shim in INTEGRATION.md, the Python
+ * cites, and a language binding (a Rust shim over this header, the Python
  * mirror in chfft_amd/) keeps the reference's method names on top of it.
  *
  * Conventions
