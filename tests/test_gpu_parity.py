@@ -315,6 +315,12 @@ def test_headline_full_batch(dtype, chunk, monkeypatch):
         y = yd.cpu().numpy()
         for b in (0, 1, 127, 128, 199, 200, 255, 256, 399, 400, 511):
             check(y[b], oracle.c2c(x[b].astype(np.complex128), m), f64, n)
+        # in place (the *i methods): every chunk's row launch overwrites only
+        # its own transforms, whose input its column launch has consumed
+        zd = xd.clone()
+        p.exec(zd, zd, m)
+        torch.cuda.synchronize()
+        assert np.array_equal(zd.cpu().numpy(), y)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
