@@ -210,6 +210,10 @@ __device__ __forceinline__ void dft_pfa(cx<T>* x) {
     for (int k2 = 0; k2 < R2; k2++) x[(A * k1 + B * k2) % R] = c[k2];
   }
 }
+struct Dft2 {
+  template <typename T>
+  __device__ __forceinline__ static void run(cx<T>* x) { dft2(x); }
+};
 struct Dft3 {
   template <typename T>
   __device__ __forceinline__ static void run(cx<T>* x) { dft3(x); }
@@ -222,7 +226,10 @@ struct Dft5 {
   template <typename T>
   __device__ __forceinline__ static void run(cx<T>* x) { dft5(x); }
 };
-// 12 = 4 x 3: k = 9 k1 + 4 k2 (mod 12); 15 = 3 x 5: k = 10 k1 + 6 k2 (mod 15)
+// 10 = 2 x 5: k = 5 k1 + 6 k2 (mod 10); 12 = 4 x 3: k = 9 k1 + 4 k2 (mod 12);
+// 15 = 3 x 5: k = 10 k1 + 6 k2 (mod 15)
+template <typename T>
+__device__ __forceinline__ void dft10(cx<T>* x) { dft_pfa<T, 2, 5, 5, 6, Dft2, Dft5>(x); }
 template <typename T>
 __device__ __forceinline__ void dft12(cx<T>* x) { dft_pfa<T, 4, 3, 9, 4, Dft4, Dft3>(x); }
 template <typename T>
@@ -236,6 +243,7 @@ __device__ __forceinline__ void dft(cx<T>* x) {
   else if constexpr (R == 5) dft5(x);
   else if constexpr (R == 7) dft7(x);
   else if constexpr (R == 8) dft8(x);
+  else if constexpr (R == 10) dft10(x);
   else if constexpr (R == 12) dft12(x);
   else if constexpr (R == 15) dft15(x);
   else if constexpr (R == 16) dft16(x);

@@ -258,29 +258,17 @@ size_t next_pow2(size_t n) {
 
 // Radix schedule for a 7-smooth L: as few passes as the radix set allows
 // (16, 8, 4, 2 for the power of two; 3, 5, 7 for the odd part). With `pfa`
-// each 3 x 5 pair becomes one prime-factor radix-15 pass and a lone 4 with a
-// 3 one radix-12 pass (180 = 12 x 15, 240 = 16 x 15): a schedule only the
-// length-specialised kernels run (the runtime-radix kernels stop at 16).
+// prime-factor radices join coprime factors into one pass: each 3 x 5 pair
+// becomes radix 15, a lone 4 with a 3 radix 12, and an odd power of two
+// (2^(4k+1)) gives its single 2 to a 5 as radix 10 (180 = 12 x 15,
+// 240 = 16 x 15, 160 = 16 x 10): a schedule only the length-specialised
+// kernels run (the runtime-radix kernels stop at 16).
 std::vector<int> radices_for(size_t L, bool pfa = false) {
-  std::vector<int> r;
   int a = 0;
   while (L % 2 == 0) {
     L /= 2;
     a++;
   }
-  int k16 = a / 4, rem = a % 4;
-  if (rem == 1 && k16 > 0) {
-    k16--;
-    r.push_back(8);
-    r.push_back(4);
-  } else if (rem == 1) {
-    r.push_back(2);
-  } else if (rem == 2) {
-    r.push_back(4);
-  } else if (rem == 3) {
-    r.push_back(8);
-  }
-  for (int i = 0; i < k16; i++) r.insert(r.begin(), 16);
   int n3 = 0, n5 = 0, n7 = 0;
   while (L % 5 == 0) {
     L /= 5;
@@ -297,6 +285,25 @@ std::vector<int> radices_for(size_t L, bool pfa = false) {
   const int n15 = pfa ? std::min(n3, n5) : 0;
   n3 -= n15;
   n5 -= n15;
+  std::vector<int> r;
+  int k16 = a / 4;
+  const int rem = a % 4;
+  const bool ten = pfa && rem == 1 && n5 > 0;  // 2 x 5
+  if (ten) {
+    n5--;
+  } else if (rem == 1 && k16 > 0) {
+    k16--;
+    r.push_back(8);
+    r.push_back(4);
+  } else if (rem == 1) {
+    r.push_back(2);
+  } else if (rem == 2) {
+    r.push_back(4);
+  } else if (rem == 3) {
+    r.push_back(8);
+  }
+  for (int i = 0; i < k16; i++) r.insert(r.begin(), 16);
+  if (ten) r.push_back(10);
   if (pfa && n3 > 0 && !r.empty() && r.back() == 4 && (r.size() == 1 || r[r.size() - 2] != 8)) {
     r.back() = 12;  // the lone 4 (not the 8 x 4 tail) joins a 3
     n3--;

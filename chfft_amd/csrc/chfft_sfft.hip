@@ -1158,7 +1158,7 @@ static const SfftEntry kSfft[] = {
     // columns as radix 8 x 15 x 3), M = 147456 = 384 x 384 (N = 65537 in
     // FP64)
     SF_BOTH(128, 16, 16, 16, true, LD_FS_CHIRP, ST_TW, 0, false, 2, 16, 8),
-    SF_BOTH(160, 16, 8, 4, false, LD_PLAIN, ST_TW, 2, false, 3, 8, 4, 5),
+    SF_BOTH(160, 16, 8, 4, false, LD_PLAIN, ST_TW, 2, false, 2, 16, 10),
     SF_BOTH(256, 16, 4, 4, false, LD_PLAIN, ST_TW, 2, false, 2, 16, 16),
     SF_BOTH(128, 16, 16, 16, true, LD_PLAIN, ST_FS_DECHIRP, 0, false, 2, 16, 8),
     SF_BOTH(384, 32, 16, 8, true, LD_FS_CHIRP, ST_TW, 0, false, 3, 16, 8, 3),
