@@ -92,8 +92,6 @@ constexpr bool tw_in_lds(int entries, int esz) {
 constexpr int lds_stride(int L, int tpb) {
   return (tpb > 1 && ((L + (L + 15) / 16) & 1) == 0) ? L + (L + 15) / 16 + 1 : L + (L + 15) / 16;
 }
-// LDS bytes of one tile: one padded buffer, or two (ping-pong) when both
-// fit in 16 KB, plus the per-pass twiddle tables when they are kept in LDS.
 // LDS (complex elements) of the MDCT fold's shared tables: the window (4L
 // reals, as 2L pairs) and the L fold twiddles, read by every transform of a
 // tile (mdct1d.rs:206-240). FP32 only: measured on MI355X (2048-point MDCT
@@ -101,6 +99,9 @@ constexpr int lds_stride(int L, int tpb) {
 // are twice the LDS), so FP64 keeps reading them from the caches.
 constexpr int mdct_tab_elems(int L, int ld, int esz) { return ld == LD_MDCT && esz == 8 ? 3 * L : 0; }
 
+// LDS bytes of one tile: one padded buffer, or two (ping-pong) when both
+// fit in 16 KB, plus the per-pass twiddle tables when they are kept in LDS,
+// plus `extra` complex elements of operator tables.
 constexpr size_t lds_need(int L, int tpb, int esz, int ntw, int extra = 0) {
   const size_t one = (size_t)tpb * lds_stride(L, tpb) * esz;
   return (2 * one <= 16384 ? 2 * one : one) + (tw_in_lds(ntw, esz) ? (size_t)ntw * esz : 0) + (size_t)extra * esz;
