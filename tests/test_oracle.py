@@ -250,3 +250,144 @@ def test_planted_mistakes_are_caught():
     x2 = uniform_complex(6000, 35).reshape(7, 5)
     assert rel(oracle.c2c2d(x2.T.copy(), "forward").T, G["fft2_7x5"]) < 1e-14
     assert rel(oracle.c2c2d(x2, "backward0"), G["fft2_7x5"]) > 0.1
+
+
+# ---------------------------------------------------------------- branches beyond the default methods
+# Each check below compares an oracle branch with something the oracle does
+# not compute: scipy/numpy transforms, mpmath closed forms, the golden direct
+# sums, or an identity, and shows that a plausible mistake in that branch
+# (a dropped factor, the other convention) would fail it.
+
+@pytest.mark.parametrize("n", [1024, 2, 6, 12, 30, 126])
+def test_dct2_ortho_scales_input_element_zero(n):
+    # dct1d.rs:220-224 and 249/251: with is_ortho the DCT-II multiplies the
+    # *input* x[0] by sqrt(1/2) before the unnormalised transform; every
+    # method then applies its own scale (1, sqrt(2/N), 2/N: dct1d.rs:98-144)
+    import scipy.fft as sfft
+    x = uniform_real(4000 + n, n)
+    xo = x.copy()
+    xo[0] *= np.sqrt(0.5)
+    ref = sfft.dct(xo, 2) / 2  # scipy's type 2 is twice chfft's unnormalised DCT-II
+    for m, s in (("forward", 1.0), ("forward0", 1.0), ("forwardu", np.sqrt(2.0 / n)), ("forwardn", 2.0 / n)):
+        assert rel(oracle.dct(x, 2, True, m), ref * s) < 1e-13, m
+    # the golden direct sum is linear in x: DCT-II(x') = DCT-II(x) - (1 - sqrt(1/2)) x0 * DCT-II(e0)
+    e0 = sfft.dct(np.eye(n)[0], 2) / 2  # = 1 for every bin
+    assert rel(oracle.dct(x, 2, True), G[f"dct2_{n}"] - (1 - np.sqrt(0.5)) * x[0] * e0) < 1e-13
+    # planted mistakes: ignoring the flag, or scaling the output bin 0 (the
+    # orthonormal convention of scipy) instead of the input element 0
+    if n > 2:
+        assert rel(oracle.dct(x, 2, False), ref) > 1e-3
+        out0 = sfft.dct(x, 2) / 2
+        out0[0] *= np.sqrt(0.5)
+        assert rel(out0, ref) > 1e-3
+
+
+@pytest.mark.parametrize("n", [4, 8, 64, 396, 2048])
+def test_vorbis_window_closed_form(n):
+    # mdct1d.rs:46-53: w(l, p) = sin(pi/2 * sin^2(pi (p + 1/2) / l)), pinned
+    # against mpmath at 30 digits, in both precisions, and the API mirror's copy
+    import mpmath
+    from chfft_amd.api import vorbis_window as api_vorbis
+    mpmath.mp.dps = 30
+    ref = np.array([float(mpmath.sin(mpmath.pi / 2 * mpmath.sin(mpmath.pi * (p + mpmath.mpf(1) / 2) / n) ** 2))
+                    for p in range(n)])
+    w = oracle.window(n, "vorbis")
+    # a few ulps: the reference evaluates the formula in T (f64 / f32)
+    assert np.max(np.abs(w - ref)) < 2e-15
+    assert np.max(np.abs(oracle.window(n, "vorbis", np.float32) - ref)) < 4e-7
+    assert np.max(np.abs(np.array([api_vorbis(n, p) for p in range(n)]) - ref)) < 2e-15
+    # Princen-Bradley: w[p]^2 + w[p + n/2]^2 = 1, the condition for TDAC
+    h = n // 2
+    assert np.max(np.abs(w[:h] ** 2 + w[h:] ** 2 - 1)) < 1e-15
+    # planted mistake: the window without the square inside (sin(pi/2 sin(.)))
+    bad = np.sin(np.pi / 2 * np.sin(np.pi * (np.arange(n) + 0.5) / n))
+    assert np.max(np.abs(bad - ref)) > 1e-2
+    assert np.max(np.abs(bad[:h] ** 2 + bad[h:] ** 2 - 1)) > 1e-2
+
+
+@pytest.mark.parametrize("kind", ["sine", "vorbis"])
+def test_mdct_tdac_reconstruction_both_windows(kind):
+    # overlap-add of backward(forward(frame)) at hop N/2 gives back the
+    # interior samples for any Princen-Bradley window (mdct1d.rs:197-291)
+    n, hop = 128, 64
+    sig = uniform_real(78, hop * 24 + n)
+    w = oracle.window(n, kind)
+    frames = (len(sig) - n) // hop + 1
+    c = oracle.mdct(sig, w, "forward", n=n, hop=hop, frames=frames)
+    y = oracle.mdct(c, w, "backward")
+    ola = np.zeros_like(sig)
+    for f in range(frames):
+        ola[f * hop:f * hop + n] += y[f]
+    inner = slice(hop, frames * hop)
+    assert rel(ola[inner], sig[inner]) < 1e-13
+    # and the windowed brute-force definitions with this window (mdct1d.rs:305-339)
+    x = uniform_real(79, n)
+    assert rel(oracle.mdct(x, w, "forward"), oracle.mdct_brute(x, w)) < 1e-12
+    cb = oracle.mdct_brute(x, w)
+    assert rel(oracle.mdct(cb, w, "backward"), oracle.imdct_brute(cb, w)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [2048, 4, 8, 12, 64, 396])
+def test_mdct_unitary_scalings(n):
+    # mdct1d.rs:149-195: forward = scaler_ui = sqrt(N/2) times the folded core,
+    # which is the unscaled definition (the golden direct sum); forwardu = 1, so
+    # forwardu = forward * sqrt(2/N) (the doc's "sqrt(2/n) scaling factor");
+    # backward = scaler_u = 1/sqrt(N/2) gives the (4/N) w sum definition;
+    # backwardu = 1, so backwardu = backward * sqrt(N/2)
+    x = uniform_real(5000 + n, n)
+    w = np.sin(np.pi * (np.arange(n) + 0.5) / n)
+    h = n // 2
+    assert rel(oracle.mdct(x, w, "forwardu"), G[f"mdct_{n}"] * np.sqrt(2.0 / n)) < 1e-12
+    c = G[f"mdct_{n}"]
+    assert rel(oracle.mdct(c, w, "backwardu"), G[f"imdct_{n}"] * np.sqrt(h)) < 1e-12
+    # the unitary pair reconstructs by TDAC too: backwardu(forwardu(.)) = backward(forward(.))
+    assert rel(oracle.mdct(oracle.mdct(x, w, "forwardu"), w, "backwardu"),
+               oracle.mdct(oracle.mdct(x, w, "forward"), w, "backward")) < 1e-12
+    # planted mistake: the reciprocal factor
+    if n > 2:
+        assert rel(oracle.mdct(x, w, "forward") / np.sqrt(2.0 / n), oracle.mdct(x, w, "forwardu")) > 0.1
+
+
+@pytest.mark.parametrize("n", [8192, 4096, 10, 6, 2])
+def test_rfft_backward_scalings(n):
+    # rfft1d.rs:153-187 and 259-312: convert_back doubles the scaler and runs
+    # the unscaled inner inverse, so backward (1/N) is numpy's irfft (which,
+    # like rfft1d.rs:164-167, ignores Im X[0] and Im X[N/2]); backward0 = N
+    # times it, backwardu = sqrt(N) times it, backwardn = backward
+    r = np.random.default_rng(n)
+    X = r.standard_normal(n // 2 + 1) + 1j * r.standard_normal(n // 2 + 1)
+    inv = np.fft.irfft(X, n)
+    for m, s in (("backward", 1.0), ("backwardn", 1.0), ("backward0", float(n)), ("backwardu", np.sqrt(n))):
+        assert rel(oracle.rfft(X, m, n=n), inv * s) < 1e-13, m
+    # planted mistake: the inner inverse without the doubling of convert_back
+    assert rel(oracle.rfft(X, "backward0", n=n) / 2, inv * n) > 0.1
+    # and the forward family against the golden numpy rfft (forward0 = forward)
+    x = uniform_real(3000 + n, n)
+    assert rel(oracle.rfft(x, "forward0"), G[f"rfft_{n}"]) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(7, 5), (1, 9), (8, 1), (4, 6), (16, 12)])
+def test_cfft2d_all_scalings_vs_numpy(shape):
+    # cfft2d.rs:150-324: forward = forward0 = 1, forwardu = 1/sqrt(MN),
+    # forwardn = 1/(MN), backward = 1/(MN), backward0 = 1, backwardu =
+    # 1/sqrt(MN); the 2-D transform is row FFTs then column FFTs (cfft2d.rs:326-363)
+    m, n = shape
+    x = uniform_complex(6100 + m * 16 + n, m * n).reshape(m, n)
+    N = m * n
+    F = np.fft.fft2(x)
+    B = np.fft.ifft2(x) * N  # unnormalised inverse
+    want = {"forward": F, "forward0": F, "forwardu": F / np.sqrt(N), "forwardn": F / N,
+            "backward": B / N, "backward0": B, "backwardu": B / np.sqrt(N)}
+    for meth, ref in want.items():
+        assert rel(oracle.c2c2d(x, meth), ref) < 1e-14, meth
+    with pytest.raises(ValueError):
+        oracle.c2c2d(x, "backwardn")  # CFft2D has no backwardn
+    # the transposed problem gives the transposed spectrum
+    assert rel(oracle.c2c2d(x.T.copy(), "forward"), F.T) < 1e-14
+    # planted mistakes: the conjugate direction, one axis left untransformed
+    if N > 2:
+        assert rel(oracle.c2c2d(x, "backward0"), F) > 0.1
+    if m > 1:
+        assert rel(np.fft.fft(x, axis=1), F) > 0.1
+    if n > 1:
+        assert rel(np.fft.fft(x, axis=0), F) > 0.1
