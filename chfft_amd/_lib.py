@@ -9,13 +9,13 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 _ROOT = os.path.dirname(_PKG)
 LIB_PATH = os.path.join(_PKG, "libchfft_hip.so")
 SOURCES = ["csrc/chfft_kernels.hip", "csrc/chfft_sfft.hip", "csrc/chfft_tables.hip", "csrc/chfft_plan.hip"]
-HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+COMPILE_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # no packed FP32 instructions (v_pk_*_f32) in any kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-OK, ERR_INVALID_LENGTH, ERR_INVALID_ARGUMENT, ERR_UNSUPPORTED, ERR_HIP, ERR_OOM = range(6)
+OK, ERR_INVALID_LENGTH, ERR_INVALID_ARGUMENT, ERR_UNSUPPORTED, ERR_HIP, ERR_OOM, ERR_INCOMPLETE = range(7)
 F32, F64 = 0, 1
 METHODS = {
     "forward": 0, "forward0": 1, "forwardu": 2, "forwardn": 3,
@@ -27,12 +27,25 @@ _LIB = None
 
 
 def build(verbose=False):
-    """Compile the HIP sources into LIB_PATH (cross-compiles without a GPU)."""
+    """Compile the HIP sources into LIB_PATH (cross-compiles without a GPU):
+    one hipcc process per source file in parallel, then one link."""
+    import concurrent.futures
+    import tempfile
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc] + HIPCC_FLAGS + [os.path.join(_PKG, s) for s in SOURCES] + ["-o", LIB_PATH + ".tmp"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stderr[-4000:])
+    with tempfile.TemporaryDirectory(prefix="chfft_build_") as tmp:
+        def compile_one(src):
+            obj = os.path.join(tmp, os.path.basename(src) + ".o")
+            cmd = [hipcc] + COMPILE_FLAGS + ["-c", os.path.join(_PKG, src), "-o", obj]
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(f"hipcc failed on {src}:\n" + r.stderr[-4000:])
+            return obj
+        with concurrent.futures.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+            objs = list(ex.map(compile_one, SOURCES))
+        r = subprocess.run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", LIB_PATH + ".tmp"],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("hipcc link failed:\n" + r.stderr[-4000:])
     os.replace(LIB_PATH + ".tmp", LIB_PATH)
     if verbose:
         print("built", LIB_PATH)
@@ -51,6 +64,8 @@ def _signatures():
         "chfft_plan_mdct_1d": ([pp, sz, vp, sz, sz, ci], ci),
         "chfft_plan_c2c_multi": ([pp, vp, sz, ci], ci),
         "chfft_plan_set_grid": ([vp, ci], ci),
+        "chfft_plan_check": ([vp, vp], ci),
+        "chfft_exec_mdct_ola": ([vp, vp, vp, ci, vp], ci),
         "chfft_plan_autotune": ([vp, vp, vp, ci, vp], ci),
         "chfft_plan_destroy": ([vp], ci),
         "chfft_exec": ([vp, vp, vp, ci, vp], ci),

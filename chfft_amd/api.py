@@ -113,23 +113,60 @@ class Plan:
             _lib.check(-r, "autotune")
         return r
 
+    def check(self, stream=0):
+        """Wait for `stream` and raise ChfftError if a multi-length launch of
+        this plan since the last check did not complete (chfft_plan_check)."""
+        _lib.check(_lib.lib().chfft_plan_check(self._h, ctypes.c_void_p(stream)), "multi-length launch")
+
     def exec_ptr(self, src, dst, method, stream=0):
         """Device pointers (ints); asynchronous on `stream` (hipStream_t as int)."""
         _lib.check(_lib.lib().chfft_exec(self._h, ctypes.c_void_p(src), ctypes.c_void_p(dst),
                                          _lib.METHODS[method], ctypes.c_void_p(stream)), "exec")
 
+    def _dtypes(self, method):
+        """(input, output) element types of `method`: complex or real, in the
+        plan's precision."""
+        back = _lib.METHODS[method] >= _lib.METHODS["backward"]
+        cin = {"c2c": True, "c2c2d": True, "c2c_multi": True, "r2c": back, "dct": False, "mdct": False}[self.kind]
+        cout = {"c2c": True, "c2c2d": True, "c2c_multi": True, "r2c": not back, "dct": False,
+                "mdct": False}[self.kind]
+        return cin, cout
+
+    def _check_tensor(self, t, complex_want, what):
+        torch = _torch()
+        want = (torch.complex128 if self.f64 else torch.complex64) if complex_want else \
+            (torch.float64 if self.f64 else torch.float32)
+        # complex data may also come as its interleaved real view
+        ok = t.dtype == want or (complex_want and t.dtype == (torch.float64 if self.f64 else torch.float32))
+        if not ok:
+            raise ValueError(f"{what}: dtype {t.dtype} does not match the plan ({want})")
+        if not t.is_contiguous():
+            raise ValueError(f"{what}: tensor must be contiguous")
+
     def exec(self, src, dst, method):
-        """numpy arrays (host, blocking) or torch CUDA tensors (device, async)."""
+        """numpy arrays (host, blocking) or torch CUDA tensors (device, async).
+        Buffers must be contiguous, of the plan's precision, and hold at least
+        the plan's batch."""
+        cin, cout = self._dtypes(method)
         if _is_torch(src):
             torch = _torch()
             if not (src.is_cuda and dst.is_cuda):
                 raise ValueError("torch tensors must live on the GPU")
+            self._check_tensor(src, cin, "input")
+            self._check_tensor(dst, cout, "output")
             ib, ob = self.io_bytes(method)
             if src.numel() * src.element_size() < ib or dst.numel() * dst.element_size() < ob:
                 raise ValueError("invalid length (buffer smaller than the plan's batch)")
             stream = torch.cuda.current_stream(src.device).cuda_stream
             self.exec_ptr(src.data_ptr(), dst.data_ptr(), method, stream)
         else:
+            fdt = np.float64 if self.f64 else np.float32
+            for a, c, what in ((src, cin, "input"), (dst, cout, "output")):
+                want = (np.complex128 if self.f64 else np.complex64) if c else fdt
+                if a.dtype != want and not (c and a.dtype == fdt):
+                    raise ValueError(f"{what}: dtype {a.dtype} does not match the plan ({np.dtype(want)})")
+                if not a.flags.c_contiguous:
+                    raise ValueError(f"{what}: array must be C-contiguous")
             ib, ob = self.io_bytes(method)
             if src.nbytes < ib or dst.nbytes < ob:
                 raise ValueError("invalid length (buffer smaller than the plan's batch)")
@@ -536,6 +573,33 @@ class Mdct1D(_Base):
             raise ValueError(f"invalid length (soure: {x.shape[-1]}, mdct.len: {self.len})")
         return self._run(x, tch, frames, hop, (frames, self.len // 2), method)
 
+    def backward_signal(self, coefs, hop, method="backward"):
+        """Inverse of forward_signal's framing: backward (or backwardu) of each
+        row of coefs [frames, len/2], overlap-added on the device at `hop`
+        (chfft_exec_mdct_ola); returns (frames - 1) * hop + len samples. With
+        hop = len/2 the samples len/2 .. (frames - 1) * hop + len/2 reproduce
+        the analysed signal (TDAC)."""
+        x, tch = self._prep(coefs, False)
+        if x.ndim != 2 or x.shape[-1] << 1 != self.len:
+            raise ValueError(f"invalid length (soure: {x.shape[-1]}, mdct.len: {self.len})")
+        frames = x.shape[0]
+        plan = self._get_plan((frames, hop), lambda: Plan("mdct", self.f64, frames, self.len, self.window, hop))
+        out = self._empty(x, ((frames - 1) * hop + self.len,), False, tch)
+        if tch:
+            stream = _torch().cuda.current_stream(x.device).cuda_stream
+            _lib.check(_lib.lib().chfft_exec_mdct_ola(plan.handle, ctypes.c_void_p(x.data_ptr()),
+                                                      ctypes.c_void_p(out.data_ptr()), _lib.METHODS[method],
+                                                      ctypes.c_void_p(stream)), "exec_mdct_ola")
+            return out
+        torch = _torch()
+        xd = torch.from_numpy(x).cuda()
+        od = torch.empty(out.shape, dtype=xd.dtype, device=xd.device)
+        _lib.check(_lib.lib().chfft_exec_mdct_ola(plan.handle, ctypes.c_void_p(xd.data_ptr()),
+                                                  ctypes.c_void_p(od.data_ptr()), _lib.METHODS[method],
+                                                  ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                   "exec_mdct_ola")
+        return od.cpu().numpy()
+
 
 class MultiFFT(_Base):
     """Many complex transforms of uneven lengths in one persistent launch
@@ -559,6 +623,10 @@ class MultiFFT(_Base):
             raise ValueError(f"invalid length (soure: {x.shape[0]}, total: {self.total})")
         out = self._empty(x, x.shape, True, tch)
         self.plan.exec(x, out, method)
+        if tch:
+            # the launch's grid-wide barriers are bounded waits: make a
+            # timed-out one an error rather than a silently incomplete output
+            self.plan.check(_torch().cuda.current_stream(x.device).cuda_stream)
         return out
 
     def forward(self, packed):

@@ -384,6 +384,113 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, const Loader<T, LD>
   }
 }
 
+// Post-processing store operators (RFft1D split, DCT-II/III and MDCT/IMDCT
+// outputs) for one transform whose final complex sequence is read through
+// Z(k) (conjugation already applied): bins i = i0, i0 + step, ... of the
+// operator's own loop, written to the transform's output at element obase.
+// Shared by the fused kernels (Z reads the LDS image) and the stand-alone
+// post-pass kernel of the long-length schedules (Z reads global memory).
+template <typename T, int ST, class ZF>
+__device__ __forceinline__ void post_ops(const KParams& P, ZF Z, void* dst, long long obase, int i0, int step, T s) {
+  if constexpr (ST == ST_R2C) {
+    // rfft1d.rs:134-148: N/2+1 bins per row. Bins i and h - i (0 < i < q)
+    // come from the same pair Z(i), Z(h - i) and twiddle co[i]: one thread
+    // computes both; i = 0 gives bins 0 and h, i = q (N % 4 == 0) bin q.
+    const int N = (int)P.N, h = N >> 1, q = (N + 3) >> 2;
+    const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
+    cx<T>* out = reinterpret_cast<cx<T>*>(dst) + obase;
+    for (int i = i0; i <= q; i += step) {
+      if (i == 0) {
+        const cx<T> z = Z(0);
+        out[0] = scl(mk(z.re + z.im, T(0)), s);
+        out[h] = scl(mk(z.re - z.im, T(0)), s);
+      } else if (i < q) {
+        const cx<T> zi = Z(i), zh = Z(h - i);
+        const cx<T> x = co[i] * (zi - cj(zh));
+        out[i] = scl(zi - x, s);
+        out[h - i] = scl(zh + cj(x), s);
+      } else if ((N & 3) == 0) {
+        out[q] = scl(cj(Z(q)), s);
+      }
+    }
+  } else if constexpr (ST == ST_DCT2) {
+    // dct1d.rs:266-287
+    const int N = (int)P.N, h = N >> 1, q2 = (N + 3) >> 2;
+    const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
+    const cx<T>* om = reinterpret_cast<const cx<T>*>(P.tab_b);
+    T* ret = reinterpret_cast<T*>(dst) + obase;
+    for (int i = i0; i <= q2; i += step) {
+      if (i == 0) {
+        const cx<T> z = Z(0);
+        ret[0] = (z.re + z.im) * s;
+        ret[h] = (z.re - z.im) * om[h].re * s;
+      } else if (i < q2) {
+        const cx<T> zi = Z(i);
+        const cx<T> wc = cj(Z(h - i));
+        const cx<T> x = co[i] * (zi - wc);
+        const cx<T> j = cj(zi - x) * om[i];
+        ret[i] = j.re * s;
+        ret[N - i] = j.im * s;
+        const cx<T> k = (wc + x) * om[h - i];
+        ret[h - i] = k.re * s;
+        ret[h + i] = k.im * s;
+      } else if ((N & 3) == 0) {
+        const cx<T> x = Z(q2) * om[q2];
+        ret[q2] = x.re * s;
+        ret[N - q2] = x.im * s;
+      }
+    }
+  } else if constexpr (ST == ST_DCT3) {
+    // dct1d.rs:395-407
+    const int N = (int)P.N, h = N >> 1, q2 = h >> 1;
+    const int per = q2 + (h & 1);
+    T* ret = reinterpret_cast<T*>(dst) + obase;
+    for (int i = i0; i < per; i += step) {
+      const cx<T> a = Z(i);
+      if (i < q2) {
+        const cx<T> b = Z(h - 1 - i);
+        ret[4 * i] = a.re * s;
+        ret[4 * i + 1] = b.im * s;
+        ret[4 * i + 2] = a.im * s;
+        ret[4 * i + 3] = b.re * s;
+      } else {
+        ret[4 * i] = a.re * s;
+        ret[4 * i + 1] = a.im * s;
+      }
+    }
+    (void)N;
+  } else if constexpr (ST == ST_MDCT) {
+    // mdct1d.rs:234-240
+    const int N = (int)P.N, q = N >> 2, h = N >> 1;
+    const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
+    T* ret = reinterpret_cast<T*>(dst) + obase;
+    for (int i = i0; i < q; i += step) {
+      const cx<T> v = Z(i) * scl(tw[i], s);
+      ret[2 * i] = -v.re;
+      ret[h - 1 - 2 * i] = v.im;
+    }
+  } else {  // ST_IMDCT, mdct1d.rs:268-288
+    const int N = (int)P.N, q = N >> 2, h = N >> 1, q3 = h + q, q5 = h + q3, lo = q - (q >> 1);
+    const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
+    const T* w = reinterpret_cast<const T*>(P.tab_b);
+    T* ret = reinterpret_cast<T*>(dst) + obase;
+    for (int i = i0; i < q; i += step) {
+      const cx<T> v = Z(i) * scl(tw[i], s);
+      const int mm = 2 * i;
+      ret[q3 - 1 - mm] = v.re * w[q3 - 1 - mm];
+      ret[q + mm] = -v.im * w[q + mm];
+      if (i < lo) {
+        ret[q3 + mm] = v.re * w[q3 + mm];
+        ret[q - 1 - mm] = v.im * w[q - 1 - mm];
+      } else {
+        ret[mm - q] = -v.re * w[mm - q];
+        ret[q5 - 1 - mm] = -v.im * w[q5 - 1 - mm];
+      }
+    }
+  }
+}
+
+
 // Store stage from the final LDS image (natural order, padded index), one
 // loop per store operator.
 template <typename T, int ST>
@@ -431,107 +538,11 @@ __device__ void store_stage(const KParams& P, const TMap& m, void* dst, const cx
   const int t = m.tr;
   if (t >= nt) return;
   const cx<T>* row = lds + t * P.Lp;
-  auto Z = [&](int k) {
-    cx<T> z = row[lx(k)];
-    return P.conj_out ? cj(z) : z;
-  };
-  const long long obase = (long long)(g0 + t) * P.out.bdist;
-  if constexpr (ST == ST_R2C) {
-    // rfft1d.rs:134-148: N/2+1 bins per row. Bins i and h - i (0 < i < q)
-    // come from the same pair Z(i), Z(h - i) and twiddle co[i]: one thread
-    // computes both; i = 0 gives bins 0 and h, i = q (N % 4 == 0) bin q.
-    const int N = (int)P.N, h = N >> 1, q = (N + 3) >> 2;
-    const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
-    cx<T>* out = reinterpret_cast<cx<T>*>(dst) + obase;
-    for (int i = m.ur; i <= q; i += m.tpt) {
-      if (i == 0) {
-        const cx<T> z = Z(0);
-        out[0] = scl(mk(z.re + z.im, T(0)), s);
-        out[h] = scl(mk(z.re - z.im, T(0)), s);
-      } else if (i < q) {
-        const cx<T> zi = Z(i), zh = Z(h - i);
-        const cx<T> x = co[i] * (zi - cj(zh));
-        out[i] = scl(zi - x, s);
-        out[h - i] = scl(zh + cj(x), s);
-      } else if ((N & 3) == 0) {
-        out[q] = scl(cj(Z(q)), s);
-      }
-    }
-  } else if constexpr (ST == ST_DCT2) {
-    // dct1d.rs:266-287
-    const int N = (int)P.N, h = N >> 1, q2 = (N + 3) >> 2;
-    const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
-    const cx<T>* om = reinterpret_cast<const cx<T>*>(P.tab_b);
-    T* ret = reinterpret_cast<T*>(dst) + obase;
-    for (int i = m.ur; i <= q2; i += m.tpt) {
-      if (i == 0) {
-        const cx<T> z = Z(0);
-        ret[0] = (z.re + z.im) * s;
-        ret[h] = (z.re - z.im) * om[h].re * s;
-      } else if (i < q2) {
-        const cx<T> zi = Z(i);
-        const cx<T> wc = cj(Z(h - i));
-        const cx<T> x = co[i] * (zi - wc);
-        const cx<T> j = cj(zi - x) * om[i];
-        ret[i] = j.re * s;
-        ret[N - i] = j.im * s;
-        const cx<T> k = (wc + x) * om[h - i];
-        ret[h - i] = k.re * s;
-        ret[h + i] = k.im * s;
-      } else if ((N & 3) == 0) {
-        const cx<T> x = Z(q2) * om[q2];
-        ret[q2] = x.re * s;
-        ret[N - q2] = x.im * s;
-      }
-    }
-  } else if constexpr (ST == ST_DCT3) {
-    // dct1d.rs:395-407
-    const int N = (int)P.N, h = N >> 1, q2 = h >> 1;
-    const int per = q2 + (h & 1);
-    T* ret = reinterpret_cast<T*>(dst) + obase;
-    for (int i = m.ur; i < per; i += m.tpt) {
-      const cx<T> a = Z(i);
-      if (i < q2) {
-        const cx<T> b = Z(h - 1 - i);
-        ret[4 * i] = a.re * s;
-        ret[4 * i + 1] = b.im * s;
-        ret[4 * i + 2] = a.im * s;
-        ret[4 * i + 3] = b.re * s;
-      } else {
-        ret[4 * i] = a.re * s;
-        ret[4 * i + 1] = a.im * s;
-      }
-    }
-    (void)N;
-  } else if constexpr (ST == ST_MDCT) {
-    // mdct1d.rs:234-240
-    const int N = (int)P.N, q = N >> 2, h = N >> 1;
-    const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
-    T* ret = reinterpret_cast<T*>(dst) + obase;
-    for (int i = m.ur; i < q; i += m.tpt) {
-      const cx<T> v = Z(i) * scl(tw[i], s);
-      ret[2 * i] = -v.re;
-      ret[h - 1 - 2 * i] = v.im;
-    }
-  } else {  // ST_IMDCT, mdct1d.rs:268-288
-    const int N = (int)P.N, q = N >> 2, h = N >> 1, q3 = h + q, q5 = h + q3, lo = q - (q >> 1);
-    const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
-    const T* w = reinterpret_cast<const T*>(P.tab_b);
-    T* ret = reinterpret_cast<T*>(dst) + obase;
-    for (int i = m.ur; i < q; i += m.tpt) {
-      const cx<T> v = Z(i) * scl(tw[i], s);
-      const int mm = 2 * i;
-      ret[q3 - 1 - mm] = v.re * w[q3 - 1 - mm];
-      ret[q + mm] = -v.im * w[q + mm];
-      if (i < lo) {
-        ret[q3 + mm] = v.re * w[q3 + mm];
-        ret[q - 1 - mm] = v.im * w[q - 1 - mm];
-      } else {
-        ret[mm - q] = -v.re * w[mm - q];
-        ret[q5 - 1 - mm] = -v.im * w[q5 - 1 - mm];
-      }
-    }
-  }
+  const bool cjo = P.conj_out != 0;
+  post_ops<T, ST>(P, [&](int k) {
+    const cx<T> z = row[lx(k)];
+    return cjo ? cj(z) : z;
+  }, dst, (long long)(g0 + t) * P.out.bdist, m.ur, m.tpt, s);
 }
 
 }  // namespace chfft

@@ -139,6 +139,9 @@ struct MultiUnit {
   int g0, g1;
 };
 constexpr int kMultiPhases = 3;
+// Bound of each grid-wide barrier wait of the multi-length launch, in ticks
+// of the 100 MHz real-time counter (4 s).
+constexpr unsigned long long kBarrierTicks = 400000000ull;
 
 // Device table generation job (chfft_tables.hip).
 struct RootJob {

@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "chfft_device.h"
 
 namespace chfft {
@@ -213,17 +215,18 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? kMaxThreadsFsF64 : kMaxThread
   }
 }
 
-// Grid-wide barrier of the multi-length launch. Every workgroup of the grid is
-// resident at once (the planner sizes the grid to the occupancy), so a
-// workgroup that arrives waits only for the others to finish their share of
-// the phase. The wait is bounded: after kBarrierTicks of the 100 MHz
-// real-time counter (4 s) the workgroup records a failure in sync[1] and
-// leaves the kernel, so no wave can wait forever. The agent-scope fences
-// publish each workgroup's scratch writes to every XCD before it arrives, and
-// make the others' writes visible after.
-constexpr unsigned long long kBarrierTicks = 400000000ull;
-
-__device__ __forceinline__ bool grid_barrier(unsigned* sync, unsigned target) {
+// Grid-wide barrier of the multi-length launch. The launch is cooperative
+// where the stream allows it (every workgroup resident, guaranteed by the
+// runtime) and otherwise sized to the occupancy, so a workgroup that arrives
+// waits only for the others to finish their share of the phase. The wait is
+// bounded: after `ticks` of the 100 MHz real-time counter (4 s by default)
+// the workgroup records the failure in sync[1], which stays set until the
+// host reads it (chfft_plan_check), and leaves the kernel, so no wave can
+// wait forever. `extra` raises the arrival target (0 in use; the host's test
+// mode sets it to make every wait time out). The agent-scope fences publish
+// each workgroup's scratch writes to every XCD before it arrives, and make
+// the others' writes visible after.
+__device__ __forceinline__ bool grid_barrier(unsigned* sync, unsigned target, unsigned long long ticks) {
   __shared__ int ok;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -233,7 +236,7 @@ __device__ __forceinline__ bool grid_barrier(unsigned* sync, unsigned target) {
     const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
     while (__hip_atomic_load(&sync[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(8);
-      if (__builtin_amdgcn_s_memrealtime() - t0 > kBarrierTicks) {
+      if (__builtin_amdgcn_s_memrealtime() - t0 > ticks) {
         good = 0;
         __hip_atomic_fetch_or(&sync[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
@@ -246,11 +249,11 @@ __device__ __forceinline__ bool grid_barrier(unsigned* sync, unsigned target) {
   return ok != 0;
 }
 
-// Clears the barrier counter and the failure flag before each multi-length
-// launch (a kernel rather than a memset, so that the pair can be captured in
-// a graph and replayed).
+// Clears the barrier counter before each multi-length launch (a kernel rather
+// than a memset, so that the pair can be captured in a graph and replayed).
+// The failure flag sync[1] is left alone: the host clears it when it reads it.
 __global__ void multi_sync_reset(unsigned* sync) {
-  if (threadIdx.x < 2) sync[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) sync[0] = 0u;
 }
 
 // Uneven multi-length batch in one persistent launch, in up to three phases
@@ -268,7 +271,7 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? kMultiThreadsF64 : kMultiThre
     __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 4)))
     multi_kernel(const MultiItem* __restrict__ items, const MultiUnit* __restrict__ units, int end0, int end1,
                  int end2, const KParams* __restrict__ descs, const void* in, void* out, int scale_kind,
-                 void* scratch, unsigned* sync) {
+                 void* scratch, unsigned* sync, unsigned long long ticks, unsigned extra) {
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
   const int ends[kMultiPhases] = {end0, end1, end2};
@@ -311,21 +314,27 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? kMultiThreadsF64 : kMultiThre
       }
     }
     u0 = ends[ph];
-    if (ph + 1 < nph && !grid_barrier(sync, (unsigned)(ph + 1) * gridDim.x)) return;
+    if (ph + 1 < nph && !grid_barrier(sync, (unsigned)(ph + 1) * gridDim.x + extra, ticks)) return;
   }
 }
 
+// coop: launch cooperatively (the runtime then refuses a grid whose
+// workgroups cannot all be resident at once); a stream under graph capture
+// takes the ordinary launch.
 hipError_t launch_multi(bool f64, const MultiItem* items, const MultiUnit* units, const int* ends,
                         const KParams* descs, const void* in, void* out, int scale_kind, void* scratch,
-                        unsigned* sync, dim3 grid, size_t lds, hipStream_t st) {
+                        unsigned* sync, unsigned long long ticks, unsigned extra, bool coop, dim3 grid, size_t lds,
+                        hipStream_t st) {
   hipLaunchKernelGGL(multi_sync_reset, dim3(1), dim3(64), 0, st, sync);
-  if (f64)
-    hipLaunchKernelGGL(multi_kernel<double>, grid, dim3(kMultiThreadsF64), lds, st, items, units, ends[0], ends[1],
-                       ends[2], descs, in, out, scale_kind, scratch, sync);
-  else
-    hipLaunchKernelGGL(multi_kernel<float>, grid, dim3(kMultiThreadsF32), lds, st, items, units, ends[0], ends[1],
-                       ends[2], descs, in, out, scale_kind, scratch, sync);
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const dim3 block(f64 ? kMultiThreadsF64 : kMultiThreadsF32);
+  const void* fn = f64 ? reinterpret_cast<const void*>(&multi_kernel<double>)
+                       : reinterpret_cast<const void*>(&multi_kernel<float>);
+  int e0 = ends[0], e1 = ends[1], e2 = ends[2];
+  void* args[] = {&items, &units, &e0, &e1, &e2, &descs, &in, &out, &scale_kind, &scratch, &sync, &ticks, &extra};
+  if (coop) return hipLaunchCooperativeKernel(fn, grid, block, args, (unsigned)lds, st);
+  return hipLaunchKernel(fn, grid, block, args, lds, st);
 }
 
 int multi_occupancy(bool f64, size_t lds) {
@@ -342,6 +351,146 @@ __global__ void copy_kernel(const T* __restrict__ src, T* __restrict__ dst, long
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// ------------------------------------------------------------------ long-length operator passes
+// RFft1D / Dct1D / Mdct1D whose half (quarter) length exceeds one
+// workgroup's LDS run their inner complex transform through the four-step
+// launches of the complex engine; the operators then become passes of their
+// own over a work buffer of P.L complex elements per transform:
+//   pre   work[g][n] = load operator element n of transform g (LD_*),
+//   post  the store operator over work[g] read from global memory (ST_*).
+// The grid's y dimension walks the transforms (grid-stride), x the elements.
+template <typename T, int LD>
+__global__ void __launch_bounds__(256) pre_kernel(KParams P) {
+  cx<T>* work = reinterpret_cast<cx<T>*>(P.dst);
+  for (int g = blockIdx.y; g < P.G; g += gridDim.y) {
+    const Loader<T, LD> ld = make_loader<T, LD>(P, P.src, g);
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < P.L; n += gridDim.x * blockDim.x) {
+      cx<T> v = load_elem<T, LD>(P, ld, n);
+      if (P.conj_in) v = cj(v);
+      work[(long long)g * P.L + n] = v;
+    }
+  }
+}
+
+template <typename T, int ST>
+__global__ void __launch_bounds__(256) post_kernel(KParams P) {
+  const cx<T>* work = reinterpret_cast<const cx<T>*>(P.src);
+  const bool cjo = P.conj_out != 0;
+  const T s = (T)P.scale;
+  for (int g = blockIdx.y; g < P.G; g += gridDim.y) {
+    const cx<T>* row = work + (long long)g * P.L;
+    post_ops<T, ST>(P, [&](int k) {
+      const cx<T> z = row[k];
+      return cjo ? cj(z) : z;
+    }, P.dst, (long long)g * P.out.bdist, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, s);
+  }
+}
+
+// Batched out-of-place transpose of complex R x C matrices (row-major,
+// matrix b at element b * R * C) through 32 x 32 LDS tiles: the column
+// pass of a 2-D transform whose column length exceeds one workgroup's LDS
+// runs as rows of the transposed matrix.
+template <typename T>
+__global__ void __launch_bounds__(256) transpose_kernel(const cx<T>* __restrict__ src, cx<T>* __restrict__ dst,
+                                                        int R, int C, int batch) {
+  __shared__ cx<T> tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+  const int tilesC = (C + 31) / 32, tilesR = (R + 31) / 32;
+  const long long per = (long long)tilesC * tilesR;
+  for (long long t = blockIdx.x; t < per * batch; t += gridDim.x) {
+    const int b = (int)(t / per);
+    const int rem = (int)(t - (long long)b * per);
+    const int r0 = (rem / tilesC) * 32, c0 = (rem % tilesC) * 32;
+    const cx<T>* s = src + (long long)b * R * C;
+    cx<T>* d = dst + (long long)b * R * C;
+    for (int i = ty; i < 32; i += 8)
+      if (r0 + i < R && c0 + tx < C) tile[i][tx] = s[(long long)(r0 + i) * C + c0 + tx];
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8)
+      if (c0 + i < C && r0 + tx < R) d[(long long)(c0 + i) * R + r0 + tx] = tile[tx][i];
+    __syncthreads();
+  }
+}
+
+// Overlap-add of `frames` windowed IMDCT frames of n samples placed `hop`
+// apart (the caller-side step after Mdct1D::backward, mdct1d.rs:244-291):
+// out[t] = sum over frames f with 0 <= t - f * hop < n of frames[f][t - f hop],
+// summed in increasing f, for t < (frames - 1) * hop + n.
+template <typename T>
+__global__ void __launch_bounds__(256) ola_kernel(const T* __restrict__ fr, T* __restrict__ out, long long n,
+                                                  long long hop, long long frames) {
+  const long long total = (frames - 1) * hop + n;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    long long f = t >= n ? (t - n) / hop + 1 : 0;  // first frame reaching t
+    const long long f1 = min(t / hop, frames - 1);
+    T acc = T(0);
+    for (; f <= f1; f++) acc += fr[f * n + (t - f * hop)];
+    out[t] = acc;
+  }
+}
+
+template <typename T>
+static hipError_t launch_pre_t(int ld, const KParams& P, hipStream_t st) {
+  const dim3 grid((unsigned)std::min((P.L + 255) / 256, 1024), (unsigned)std::min(P.G, 65535));
+  switch (ld) {
+    case LD_PLAIN: hipLaunchKernelGGL((pre_kernel<T, LD_PLAIN>), grid, dim3(256), 0, st, P); break;
+    case LD_R2C_BACK: hipLaunchKernelGGL((pre_kernel<T, LD_R2C_BACK>), grid, dim3(256), 0, st, P); break;
+    case LD_DCT2: hipLaunchKernelGGL((pre_kernel<T, LD_DCT2>), grid, dim3(256), 0, st, P); break;
+    case LD_DCT3: hipLaunchKernelGGL((pre_kernel<T, LD_DCT3>), grid, dim3(256), 0, st, P); break;
+    case LD_MDCT: hipLaunchKernelGGL((pre_kernel<T, LD_MDCT>), grid, dim3(256), 0, st, P); break;
+    case LD_IMDCT: hipLaunchKernelGGL((pre_kernel<T, LD_IMDCT>), grid, dim3(256), 0, st, P); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_post_t(int st_op, const KParams& P, hipStream_t st) {
+  // every operator loops over at most L / 2 + 1 bins
+  const dim3 grid((unsigned)std::min((P.L / 2 + 256) / 256, 1024), (unsigned)std::min(P.G, 65535));
+  switch (st_op) {
+    case ST_R2C: hipLaunchKernelGGL((post_kernel<T, ST_R2C>), grid, dim3(256), 0, st, P); break;
+    case ST_DCT2: hipLaunchKernelGGL((post_kernel<T, ST_DCT2>), grid, dim3(256), 0, st, P); break;
+    case ST_DCT3: hipLaunchKernelGGL((post_kernel<T, ST_DCT3>), grid, dim3(256), 0, st, P); break;
+    case ST_MDCT: hipLaunchKernelGGL((post_kernel<T, ST_MDCT>), grid, dim3(256), 0, st, P); break;
+    case ST_IMDCT: hipLaunchKernelGGL((post_kernel<T, ST_IMDCT>), grid, dim3(256), 0, st, P); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pre(bool f64, int ld, const KParams& P, hipStream_t st) {
+  return f64 ? launch_pre_t<double>(ld, P, st) : launch_pre_t<float>(ld, P, st);
+}
+
+hipError_t launch_post(bool f64, int st_op, const KParams& P, hipStream_t st) {
+  return f64 ? launch_post_t<double>(st_op, P, st) : launch_post_t<float>(st_op, P, st);
+}
+
+hipError_t launch_transpose(bool f64, const void* src, void* dst, int R, int C, int batch, hipStream_t st) {
+  const long long tiles = (long long)((R + 31) / 32) * ((C + 31) / 32) * batch;
+  const dim3 grid((unsigned)std::min<long long>(tiles, 65536));
+  if (f64)
+    hipLaunchKernelGGL(transpose_kernel<double>, grid, dim3(256), 0, st, (const cx<double>*)src, (cx<double>*)dst, R,
+                       C, batch);
+  else
+    hipLaunchKernelGGL(transpose_kernel<float>, grid, dim3(256), 0, st, (const cx<float>*)src, (cx<float>*)dst, R, C,
+                       batch);
+  return hipGetLastError();
+}
+
+hipError_t launch_ola(bool f64, const void* frames, void* out, long long n, long long hop, long long count,
+                      hipStream_t st) {
+  const long long total = (count - 1) * hop + n;
+  const dim3 grid((unsigned)std::min<long long>((total + 255) / 256, 65536));
+  if (f64)
+    hipLaunchKernelGGL(ola_kernel<double>, grid, dim3(256), 0, st, (const double*)frames, (double*)out, n, hop, count);
+  else
+    hipLaunchKernelGGL(ola_kernel<float>, grid, dim3(256), 0, st, (const float*)frames, (float*)out, n, hop, count);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ host-side launch table

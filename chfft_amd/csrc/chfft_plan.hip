@@ -43,7 +43,13 @@ hipError_t launch_fourstep(bool f64, int nph, const KParams* Q, const void* in, 
 int fourstep_occupancy(bool f64, int nph, int block, size_t lds);
 hipError_t launch_multi(bool f64, const MultiItem* items, const MultiUnit* units, const int* ends,
                         const KParams* descs, const void* in, void* out, int scale_kind, void* scratch,
-                        unsigned* sync, dim3 grid, size_t lds, hipStream_t st);
+                        unsigned* sync, unsigned long long ticks, unsigned extra, bool coop, dim3 grid, size_t lds,
+                        hipStream_t st);
+hipError_t launch_pre(bool f64, int ld, const KParams& P, hipStream_t st);
+hipError_t launch_post(bool f64, int st_op, const KParams& P, hipStream_t st);
+hipError_t launch_transpose(bool f64, const void* src, void* dst, int R, int C, int batch, hipStream_t st);
+hipError_t launch_ola(bool f64, const void* frames, void* out, long long n, long long hop, long long count,
+                      hipStream_t st);
 int multi_occupancy(bool f64, size_t lds);
 hipError_t gen_roots(bool f64, const RootJob* jobs_dev, int njobs, void* arena, hipStream_t st);
 hipError_t gen_rot(const long long* ns_dev, int count, long long M, void* out, hipStream_t st);
@@ -374,11 +380,17 @@ Geometry choose_geometry(int Lfft, long long G, bool colmajor, bool f64, int max
 
 // ------------------------------------------------------------------ plan objects
 enum Kind { K_C2C = 0, K_R2C = 1, K_C2C2D = 2, K_DCT = 3, K_MDCT = 4, K_MULTI = 5 };
-enum Slot { S_IN = 0, S_OUT = 1, S_SCRATCH = 2 };
+// Buffers a launch reads and writes: the exec's input and output, the
+// plan's four-step scratch, and the plan's work buffer (the operand of the
+// long-length operator passes and of the transposed 2-D column pass).
+enum Slot { S_IN = 0, S_OUT = 1, S_SCRATCH = 2, S_WORK = 3 };
+constexpr int kSlots = 4;
 
 struct Launch {
   int kind = 0;  // 0 fused kernel, 1 copy, 2 persistent four-step kernel, 3 specialised fused kernel,
-                 // 4 specialised persistent four-step kernel
+                 // 4 specialised persistent four-step kernel, 5 operator pre-pass (P.L elements per
+                 // transform into the work layout), 6 operator post-pass, 7 transpose (P.G matrices
+                 // of P.Lfft x P.L complex elements)
   const SfftEntry* sf = nullptr;
   const SfsEntry* sfs = nullptr;
   int ld, st;
@@ -1078,6 +1090,9 @@ struct chfft_plan_s {
   Inner inner;           // r2c / dct / mdct
   DevBuf tab_a, tab_b, tab_c;
   DevBuf scratch;
+  DevBuf work;            // S_WORK: operator passes / transposed 2-D columns
+  DevBuf ola;             // MDCT: windowed frames before the overlap-add
+  bool long_inner = false;  // r2c / dct / mdct inner transform on the four-step engine (rows)
   DevBuf stage_in, stage_out;
   std::vector<std::unique_ptr<DevBuf>> qbufs;  // four-step phase descriptors
   std::vector<Launch> fwd, bwd;
@@ -1089,6 +1104,8 @@ struct chfft_plan_s {
   int unit_ends[kMultiPhases] = {0, 0, 0};  // units of phases 0..2 end here
   size_t multi_lds = 0;
   int multi_grid = 0, multi_occ = 1;
+  unsigned long long barrier_ticks = kBarrierTicks;
+  unsigned barrier_extra = 0;
   std::string multi_desc;
 };
 
@@ -1129,10 +1146,73 @@ Launch fused_op(const chfft_plan_s* p, int ld, int st, bool inner_inverse, long 
   return l;
 }
 
+// A stand-alone operator pass of a long-length schedule (kind 5: load
+// operator `op` into the work layout; kind 6: store operator `op` from it):
+// G transforms of L complex work elements, the operator side at in_dist /
+// out_dist elements per transform.
+Launch op_pass(int kind, int op, size_t L, long long in_dist, long long out_dist, int src, int dst, size_t G) {
+  Launch l{};
+  l.kind = kind;
+  l.ld = kind == 5 ? op : LD_PLAIN;
+  l.st = kind == 6 ? op : ST_PLAIN;
+  l.P.G = (int)G;
+  l.P.L = l.P.Lfft = (int)L;
+  l.P.in = io(in_dist, 0, 1, 1);
+  l.P.out = io(out_dist, 0, 1, 1);
+  l.src = src;
+  l.dst = dst;
+  l.scaled = kind == 6 ? 1 : 0;
+  l.scale_mul = 1.0;
+  static const char* const lds[] = {"plain", "chirp", "RFft un-split", "DCT-II pack", "DCT-III pre-twiddle",
+                                    "MDCT fold", "IMDCT pre-twiddle"};
+  static const char* const sts[] = {"plain", "twiddle", "de-chirp", "RFft split", "DCT-II post-twiddle",
+                                    "DCT-III unpack", "MDCT post-twiddle", "IMDCT unfold"};
+  l.what = std::string(kind == 5 ? "operator pre-pass " : "operator post-pass ") + (kind == 5 ? lds[op] : sts[op]) +
+           " L=" + std::to_string(L);
+  return l;
+}
+
+// The inner complex transform of a long-length plan: `batch` transforms of
+// h points on the complex engine (p->rows) between the given slots; its
+// launches apply no scale (the operator post-pass or the caller does).
+int long_inner_init(chfft_plan_s* p, size_t h, size_t batch) {
+  int rc = p->rows.init(h, p->f64);
+  if (rc) return rc;
+  p->rows.force_split = true;
+  p->rows.force_persistent = false;
+  p->long_inner = true;
+  return dev_alloc(p->work, batch * h * esz_of(p->f64));
+}
+
+void emit_unscaled(const chfft_plan_s* p, std::vector<Launch>& L, size_t batch, bool inverse, int src, int dst,
+                   long long dist) {
+  const size_t first = L.size();
+  p->rows.emit(L, batch, inverse, src, dst, dist, dist);
+  for (size_t i = first; i < L.size(); i++) L[i].scaled = 0;
+}
+
+Launch transpose_launch(int src, int dst, size_t R, size_t C, size_t batch) {
+  Launch l{};
+  l.kind = 7;
+  l.P.G = (int)batch;
+  l.P.Lfft = (int)R;
+  l.P.L = (int)C;
+  l.src = src;
+  l.dst = dst;
+  l.scaled = 0;
+  l.scale_mul = 1.0;
+  l.what = "transpose " + std::to_string(R) + " x " + std::to_string(C);
+  return l;
+}
+
 int finish_plan(chfft_plan_s* p, chfft_plan* out) {
   size_t need = 0;
   if (p->kind == K_C2C) need = p->rows.scratch_elems(p->batch);
-  if (p->kind == K_C2C2D) need = p->rows.scratch_elems(p->batch * p->m);
+  if (p->kind == K_C2C2D) {
+    need = p->rows.scratch_elems(p->batch * p->m);
+    if (p->cols.mode >= 2) need = std::max(need, p->cols.scratch_elems(p->batch * p->n));
+  }
+  if (p->long_inner) need = p->rows.scratch_elems(p->batch);
   need = std::max(need, p->rows.persistent_scratch_elems(p->fwd));
   need = std::max(need, p->rows.persistent_scratch_elems(p->bwd));
   for (std::vector<Launch>* L : {&p->fwd, &p->bwd})
@@ -1171,6 +1251,7 @@ const char* chfft_status_string(int status) {
     case CHFFT_ERR_UNSUPPORTED: return "unsupported";
     case CHFFT_ERR_HIP: return "HIP runtime error";
     case CHFFT_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case CHFFT_ERR_INCOMPLETE: return "incomplete: a grid-wide barrier of the multi-length launch timed out";
   }
   return "unknown status";
 }
@@ -1217,21 +1298,41 @@ int chfft_plan_c2c_2d(chfft_plan* plan, size_t len_m, size_t len_n, size_t batch
   }
   p->rows.force_split = true;  // the column pass below continues from split launches
   p->rows.force_persistent = false;
-  if (len_m > 1 && !fusable(len_m, p->f64)) {
-    delete p;
-    return CHFFT_ERR_UNSUPPORTED;
-  }
-  if (len_m > 1 && (rc = build_inner(len_m, p->f64, p->inner))) {
+  // columns: one fused launch in place on `out` when the column length fits
+  // one workgroup; otherwise the matrices are transposed into the work
+  // buffer, the columns run as rows on the complex engine, and the result
+  // is transposed back
+  const bool col_fused = len_m > 1 && fusable(len_m, p->f64);
+  const bool col_long = len_m > 1 && !col_fused;
+  if (col_fused && (rc = build_inner(len_m, p->f64, p->inner))) {
     delete p;
     return rc;
   }
+  if (col_long) {
+    if ((rc = p->cols.init(len_m, p->f64)) ||
+        (rc = dev_alloc(p->work, batch * len_m * len_n * esz_of(p->f64)))) {
+      delete p;
+      return rc;
+    }
+    p->cols.force_split = true;
+    p->cols.force_persistent = false;
+  }
   for (int dir = 0; dir < 2; dir++) {
     std::vector<Launch>& L = dir ? p->bwd : p->fwd;
-    // rows: batch * m transforms of length n (cfft2d.rs:340-348)
+    // rows: batch * m transforms of length n (cfft2d.rs:340-348); with a
+    // column pass to follow they leave the scaling to it and, in an
+    // inverse, conjugate on load only (the column pass conjugates on store)
     p->rows.emit(L, batch * len_m, dir == 1, S_IN, S_OUT, (long long)len_n, (long long)len_n);
-    for (auto& l : L) l.scaled = 0;
-    // columns: batch * n transforms of length m, stride n, in place on out
-    if (len_m > 1) {
+    if (len_m > 1)
+      for (auto& l : L) {
+        l.scaled = 0;
+        if (dir) l.P.conj_out = 0;
+      }
+    // an inverse row pass leaves its output conjugated; a length-1 row (plain
+    // copy) does not, so the column pass then conjugates on load
+    const int cin = p->rows.mode == 0 ? dir : 0;
+    if (col_fused) {
+      // columns: batch * n transforms of length m, stride n, in place on out
       Launch c{};
       c.ld = LD_PLAIN;
       c.st = ST_PLAIN;
@@ -1241,24 +1342,23 @@ int chfft_plan_c2c_2d(chfft_plan* plan, size_t len_m, size_t len_n, size_t batch
       c.P.out = c.P.in;
       c.P.colmajor_ld = c.P.colmajor_st = 1;
       c.P.conj_out = dir;
-      // an inverse row pass leaves its output conjugated; a length-1 row
-      // (plain copy) does not
-      c.P.conj_in = p->rows.mode == 0 ? dir : 0;
+      c.P.conj_in = cin;
       Geometry g = choose_geometry((int)p->inner.Lfft, c.P.G, true, p->f64);
       C2CEngine::finish(c, g, S_OUT, S_OUT, 1);
       c.what = "2-D columns " + seq_str(p->inner.seq);
       try_specialise(c, p->f64);
       L.push_back(c);
-    } else {
-      // a single row: undo the conjugation and scale with a plain pass
-      Launch& last = L.back();
-      last.scaled = 1;
+    } else if (col_long) {
+      L.push_back(transpose_launch(S_OUT, S_WORK, len_m, len_n, batch));
+      const size_t first = L.size();
+      p->cols.emit(L, batch * len_n, dir == 1, S_WORK, S_WORK, (long long)len_m, (long long)len_m);
+      if (dir && !cin)
+        for (size_t i = first; i < L.size(); i++)
+          if (L[i].src == S_WORK) L[i].P.conj_in = 0;
+      for (size_t i = first; i < L.size(); i++) L[i].what = "2-D columns (transposed) " + L[i].what;
+      L.push_back(transpose_launch(S_WORK, S_OUT, len_n, len_m, batch));
     }
   }
-  // row pass of an inverse: conjugate on load only, keep conjugated output
-  // (every launch but the last, the column pass)
-  if (len_m > 1)
-    for (size_t i = 0; i + 1 < p->bwd.size(); i++) p->bwd[i].P.conj_out = 0;
   return finish_plan(p, plan);
 }
 
@@ -1272,13 +1372,13 @@ int chfft_plan_r2c_1d(chfft_plan* plan, size_t len, size_t batch, int precision)
   if (rc) return rc;
   const bool f64 = precision == CHFFT_F64;
   const size_t h = len / 2;
-  if (!fusable(h, f64)) return CHFFT_ERR_UNSUPPORTED;
+  const bool longp = !fusable(h, f64);
   chfft_plan_s* p = new chfft_plan_s();
   p->kind = K_R2C;
   p->f64 = f64;
   p->n = len;
   p->batch = batch;
-  if ((rc = build_inner(h, f64, p->inner))) {
+  if ((rc = longp ? long_inner_init(p, h, batch) : build_inner(h, f64, p->inner))) {
     delete p;
     return rc;
   }
@@ -1296,6 +1396,25 @@ int chfft_plan_r2c_1d(chfft_plan* plan, size_t len, size_t batch, int precision)
   if ((rc = upload(co, f64, p->tab_a)) || (rc = upload(bc, f64, p->tab_b))) {
     delete p;
     return rc;
+  }
+  if (longp) {
+    // forward: the inner transform reads the len reals as h complex values
+    // into the work buffer, the split runs as a post-pass
+    emit_unscaled(p, p->fwd, batch, false, S_IN, S_WORK, (long long)h);
+    Launch f = op_pass(6, ST_R2C, h, (long long)h, (long long)(h + 1), S_WORK, S_OUT, batch);
+    f.P.N = (long long)len;
+    f.P.tab_a = p->tab_a.p;
+    p->fwd.push_back(f);
+    // backward: the un-split as a pre-pass, the inverse inner transform
+    // writes the reals and applies the scale (times 2, rfft1d.rs:161)
+    Launch b = op_pass(5, LD_R2C_BACK, h, (long long)(h + 1), (long long)h, S_IN, S_WORK, batch);
+    b.P.N = (long long)len;
+    b.P.tab_a = p->tab_b.p;
+    p->bwd.push_back(b);
+    const size_t first = p->bwd.size();
+    p->rows.emit(p->bwd, batch, true, S_WORK, S_OUT, (long long)h, (long long)h);
+    for (size_t i = first; i < p->bwd.size(); i++) p->bwd[i].scale_mul = 2.0;
+    return finish_plan(p, plan);
   }
   Launch f = fused_op(p, LD_PLAIN, ST_R2C, false, (long long)h, (long long)(h + 1), (int)batch);
   f.P.N = (long long)len;
@@ -1320,7 +1439,7 @@ int chfft_plan_dct_1d(chfft_plan* plan, int dct_type, size_t len, int is_ortho, 
   if (rc) return rc;
   const bool f64 = precision == CHFFT_F64;
   const size_t h = len / 2;
-  if (!fusable(h, f64)) return CHFFT_ERR_UNSUPPORTED;
+  const bool longp = !fusable(h, f64);
   chfft_plan_s* p = new chfft_plan_s();
   p->kind = K_DCT;
   p->f64 = f64;
@@ -1328,7 +1447,7 @@ int chfft_plan_dct_1d(chfft_plan* plan, int dct_type, size_t len, int is_ortho, 
   p->batch = batch;
   p->dct_type = dct_type;
   p->ortho = is_ortho != 0;
-  if ((rc = build_inner(h, f64, p->inner))) {
+  if ((rc = longp ? long_inner_init(p, h, batch) : build_inner(h, f64, p->inner))) {
     delete p;
     return rc;
   }
@@ -1355,13 +1474,30 @@ int chfft_plan_dct_1d(chfft_plan* plan, int dct_type, size_t len, int is_ortho, 
     return rc;
   }
   const bool two = dct_type == CHFFT_DCT_II;
+  // dct1d.rs:51-55 and 174-178
+  const double ortho = two ? (p->ortho ? std::sqrt(0.5) : 1.0) : (p->ortho ? std::sqrt(0.5) : 0.5);
+  if (longp) {
+    // pack / pre-twiddle pass, inner transform on the work buffer (inverse
+    // for DCT-III), post-twiddle / unpack pass
+    Launch a = op_pass(5, two ? LD_DCT2 : LD_DCT3, h, (long long)len, (long long)h, S_IN, S_WORK, batch);
+    Launch c = op_pass(6, two ? ST_DCT2 : ST_DCT3, h, (long long)h, (long long)len, S_WORK, S_OUT, batch);
+    for (Launch* l : {&a, &c}) {
+      l->P.N = (long long)len;
+      l->P.tab_a = p->tab_a.p;
+      l->P.tab_b = p->tab_b.p;
+      l->P.ortho = ortho;
+    }
+    p->fwd.push_back(a);
+    emit_unscaled(p, p->fwd, batch, !two, S_WORK, S_WORK, (long long)h);
+    p->fwd.push_back(c);
+    return finish_plan(p, plan);
+  }
   Launch f = fused_op(p, two ? LD_DCT2 : LD_DCT3, two ? ST_DCT2 : ST_DCT3, !two, (long long)len, (long long)len,
                       (int)batch);
   f.P.N = (long long)len;
   f.P.tab_a = p->tab_a.p;
   f.P.tab_b = p->tab_b.p;
-  // dct1d.rs:51-55 and 174-178
-  f.P.ortho = two ? (p->ortho ? std::sqrt(0.5) : 1.0) : (p->ortho ? std::sqrt(0.5) : 0.5);
+  f.P.ortho = ortho;
   p->fwd.push_back(f);
   return finish_plan(p, plan);
 }
@@ -1378,14 +1514,20 @@ int chfft_plan_mdct_1d(chfft_plan* plan, size_t len, const double* window, size_
   if (rc) return rc;
   const bool f64 = precision == CHFFT_F64;
   const size_t q = len / 4;
-  if (!fusable(q, f64)) return CHFFT_ERR_UNSUPPORTED;
+  const bool longp = !fusable(q, f64);
   chfft_plan_s* p = new chfft_plan_s();
   p->kind = K_MDCT;
   p->f64 = f64;
   p->n = len;
   p->batch = batch;
   p->frame_stride = frame_stride ? frame_stride : len;
-  if ((rc = build_inner(q, f64, p->inner))) {
+  if ((rc = longp ? long_inner_init(p, q, batch) : build_inner(q, f64, p->inner))) {
+    delete p;
+    return rc;
+  }
+  // frames of an overlapped (or spaced) stream are overlap-added by
+  // chfft_exec_mdct_ola from this buffer
+  if (p->frame_stride != len && (rc = dev_alloc(p->ola, batch * len * (f64 ? 8 : 4)))) {
     delete p;
     return rc;
   }
@@ -1402,6 +1544,26 @@ int chfft_plan_mdct_1d(chfft_plan* plan, size_t len, const double* window, size_
   if ((rc = upload(tw, f64, p->tab_a)) || (rc = upload(w, f64, p->tab_b))) {
     delete p;
     return rc;
+  }
+  if (longp) {
+    // fold + pre-twiddle, inner forward transform, post-twiddle (both
+    // directions: the IMDCT also runs the forward transform, mdct1d.rs:266)
+    Launch a = op_pass(5, LD_MDCT, q, (long long)p->frame_stride, (long long)q, S_IN, S_WORK, batch);
+    Launch c = op_pass(6, ST_MDCT, q, (long long)q, (long long)(len / 2), S_WORK, S_OUT, batch);
+    Launch d = op_pass(5, LD_IMDCT, q, (long long)(len / 2), (long long)q, S_IN, S_WORK, batch);
+    Launch e = op_pass(6, ST_IMDCT, q, (long long)q, (long long)len, S_WORK, S_OUT, batch);
+    for (Launch* l : {&a, &c, &d, &e}) {
+      l->P.N = (long long)len;
+      l->P.tab_a = p->tab_a.p;
+      l->P.tab_b = p->tab_b.p;
+    }
+    p->fwd.push_back(a);
+    emit_unscaled(p, p->fwd, batch, false, S_WORK, S_WORK, (long long)q);
+    p->fwd.push_back(c);
+    p->bwd.push_back(d);
+    emit_unscaled(p, p->bwd, batch, false, S_WORK, S_WORK, (long long)q);
+    p->bwd.push_back(e);
+    return finish_plan(p, plan);
   }
   Launch f = fused_op(p, LD_MDCT, ST_MDCT, false, (long long)p->frame_stride, (long long)(len / 2), (int)batch);
   f.P.N = (long long)len;
@@ -1515,52 +1677,64 @@ void io_bytes(const chfft_plan_s* p, bool inverse, size_t* ib, size_t* ob) {
 
 extern "C" {
 
-int chfft_exec_range(chfft_plan plan, const void* in, void* out, int method, int first, int count, void* stream) {
+}  // extern "C"
+
+namespace chfft {
+namespace {
+
+// Runs launches [first, first + count) of `method`'s schedule with the given
+// scratch and work buffers (the plan's own, or a caller's).
+int run_launches(chfft_plan plan, const void* in, void* out, void* scratch, void* work, int method, int first,
+                 int count, hipStream_t st) {
   if (!plan || !in || !out) return CHFFT_ERR_INVALID_ARGUMENT;
   bool inverse;
   double scale;
   int rc = method_scale(plan, method, &inverse, &scale);
   if (rc) return rc;
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (plan->kind == K_MULTI) {
     if (first != 0 || count != 1) return count == 0 ? CHFFT_OK : CHFFT_ERR_INVALID_ARGUMENT;
     if (plan->nitems == 0) return CHFFT_OK;
     const KParams* descs = reinterpret_cast<const KParams*>(inverse ? plan->descs_bwd.p : plan->descs_fwd.p);
-    if (launch_multi(plan->f64, reinterpret_cast<const MultiItem*>(plan->items_dev.p),
-                     reinterpret_cast<const MultiUnit*>(plan->units_dev.p), plan->unit_ends, descs, in, out,
-                     (int)scale, plan->scratch.p, reinterpret_cast<unsigned*>(plan->sync_dev.p),
-                     dim3((unsigned)plan->multi_grid), plan->multi_lds, st) != hipSuccess)
-      return CHFFT_ERR_HIP;
-    return CHFFT_OK;
+    // a cooperative launch guarantees that every workgroup is resident for
+    // the grid-wide barriers; a stream under capture takes the ordinary
+    // launch (the barrier waits stay bounded and chfft_plan_check reports a
+    // timed-out one)
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const bool coop = hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone;
+    (void)hipGetLastError();
+    auto go = [&](bool c) {
+      return launch_multi(plan->f64, reinterpret_cast<const MultiItem*>(plan->items_dev.p),
+                          reinterpret_cast<const MultiUnit*>(plan->units_dev.p), plan->unit_ends, descs, in, out,
+                          (int)scale, scratch, reinterpret_cast<unsigned*>(plan->sync_dev.p), plan->barrier_ticks,
+                          plan->barrier_extra, c, dim3((unsigned)plan->multi_grid), plan->multi_lds, st);
+    };
+    hipError_t e = go(coop);
+    if (e != hipSuccess && coop) {
+      (void)hipGetLastError();
+      e = go(false);
+    }
+    return e == hipSuccess ? CHFFT_OK : CHFFT_ERR_HIP;
   }
   const std::vector<Launch>& L = inverse ? plan->bwd : plan->fwd;
   if (first < 0 || count < 0 || (size_t)first + (size_t)count > L.size()) return CHFFT_ERR_INVALID_ARGUMENT;
   for (int li = first; li < first + count; li++) {
     const Launch& l0 = L[li];
-    const void* slots_src[3] = {in, out, plan->scratch.p};
-    void* slots_dst[3] = {const_cast<void*>(in), out, plan->scratch.p};
+    const void* slots_src[kSlots] = {in, out, scratch, work};
+    void* slots_dst[kSlots] = {const_cast<void*>(in), out, scratch, work};
     if (l0.src_off) slots_src[l0.src] = static_cast<const char*>(slots_src[l0.src]) + l0.src_off;
     if (l0.dst_off) slots_dst[l0.dst] = static_cast<char*>(slots_dst[l0.dst]) + l0.dst_off;
     if (l0.kind == 2) {
       if (launch_fourstep(plan->f64, l0.nph, reinterpret_cast<const KParams*>(l0.qdev), slots_src[l0.src],
-                          slots_dst[l0.dst], scale * l0.scale_mul, l0.batch, l0.src_dist, l0.dst_dist,
-                          plan->scratch.p, l0.slot_elems, l0.grid, l0.block, l0.lds, st) != hipSuccess)
+                          slots_dst[l0.dst], scale * l0.scale_mul, l0.batch, l0.src_dist, l0.dst_dist, scratch,
+                          l0.slot_elems, l0.grid, l0.block, l0.lds, st) != hipSuccess)
         return CHFFT_ERR_HIP;
       continue;
     }
     if (l0.kind == 4) {
       if (sfs_launch(l0.sfs, reinterpret_cast<const KParams*>(l0.qdev), slots_src[l0.src], slots_dst[l0.dst],
-                     scale * l0.scale_mul, l0.batch, l0.src_dist, l0.dst_dist, plan->scratch.p, l0.slot_elems,
-                     l0.grid, st) != hipSuccess)
+                     scale * l0.scale_mul, l0.batch, l0.src_dist, l0.dst_dist, scratch, l0.slot_elems, l0.grid,
+                     st) != hipSuccess)
         return CHFFT_ERR_HIP;
-      continue;
-    }
-    if (l0.kind == 3) {
-      KParams P = l0.P;
-      P.src = slots_src[l0.src];
-      P.dst = slots_dst[l0.dst];
-      P.scale = l0.scaled ? scale * l0.scale_mul : 1.0;
-      if (sfft_launch(l0.sf, P, l0.grid, l0.lds, st) != hipSuccess) return CHFFT_ERR_HIP;
       continue;
     }
     if (l0.kind == 1) {  // length-1 copy
@@ -1571,14 +1745,75 @@ int chfft_exec_range(chfft_plan plan, const void* in, void* out, int method, int
         return CHFFT_ERR_HIP;
       continue;
     }
+    if (l0.kind == 7) {
+      if (launch_transpose(plan->f64, slots_src[l0.src], slots_dst[l0.dst], l0.P.Lfft, l0.P.L, l0.P.G, st) !=
+          hipSuccess)
+        return CHFFT_ERR_HIP;
+      continue;
+    }
     KParams P = l0.P;
     P.src = slots_src[l0.src];
     P.dst = slots_dst[l0.dst];
     P.scale = l0.scaled ? scale * l0.scale_mul : 1.0;
-    if (launch_fused(plan->f64, l0.ld, l0.st, P, l0.grid, l0.block, l0.lds, st) != hipSuccess)
-      return CHFFT_ERR_HIP;
+    hipError_t e;
+    if (l0.kind == 3)
+      e = sfft_launch(l0.sf, P, l0.grid, l0.lds, st);
+    else if (l0.kind == 5)
+      e = launch_pre(plan->f64, l0.ld, P, st);
+    else if (l0.kind == 6)
+      e = launch_post(plan->f64, l0.st, P, st);
+    else
+      e = launch_fused(plan->f64, l0.ld, l0.st, P, l0.grid, l0.block, l0.lds, st);
+    if (e != hipSuccess) return CHFFT_ERR_HIP;
   }
   return CHFFT_OK;
+}
+
+// Reads and clears the multi-length launch's sticky barrier-failure flag
+// (stream-ordered after `st`, blocking).
+int read_barrier_flag(chfft_plan plan, hipStream_t st) {
+  if (hipStreamSynchronize(st) != hipSuccess) return CHFFT_ERR_HIP;
+  if (plan->kind != K_MULTI || !plan->sync_dev.p) return CHFFT_OK;
+  unsigned flag = 0;
+  unsigned* f = reinterpret_cast<unsigned*>(plan->sync_dev.p) + 1;
+  if (hipMemcpy(&flag, f, sizeof flag, hipMemcpyDeviceToHost) != hipSuccess) return CHFFT_ERR_HIP;
+  if (!flag) return CHFFT_OK;
+  if (hipMemset(f, 0, sizeof flag) != hipSuccess) return CHFFT_ERR_HIP;
+  return CHFFT_ERR_INCOMPLETE;
+}
+
+}  // namespace
+}  // namespace chfft
+
+extern "C" {
+
+int chfft_exec_range(chfft_plan plan, const void* in, void* out, int method, int first, int count, void* stream) {
+  if (!plan) return CHFFT_ERR_INVALID_ARGUMENT;
+  return run_launches(plan, in, out, plan->scratch.p, plan->work.p, method, first, count,
+                      reinterpret_cast<hipStream_t>(stream));
+}
+
+int chfft_plan_check(chfft_plan plan, void* stream) {
+  if (!plan) return CHFFT_ERR_INVALID_ARGUMENT;
+  return read_barrier_flag(plan, reinterpret_cast<hipStream_t>(stream));
+}
+
+int chfft_exec_mdct_ola(chfft_plan plan, const void* in, void* out, int method, void* stream) {
+  if (!plan || !in || !out || plan->kind != K_MDCT) return CHFFT_ERR_INVALID_ARGUMENT;
+  bool inverse;
+  double scale;
+  int rc = method_scale(plan, method, &inverse, &scale);
+  if (rc) return rc;
+  if (!inverse) return CHFFT_ERR_INVALID_ARGUMENT;  // only the backward methods give frames to overlap-add
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const int n = (int)plan->bwd.size();
+  if (plan->frame_stride == plan->n)  // frames back to back: the frames are the signal
+    return run_launches(plan, in, out, plan->scratch.p, plan->work.p, method, 0, n, st);
+  if ((rc = run_launches(plan, in, plan->ola.p, plan->scratch.p, plan->work.p, method, 0, n, st))) return rc;
+  return launch_ola(plan->f64, plan->ola.p, out, (long long)plan->n, (long long)plan->frame_stride,
+                    (long long)plan->batch, st) == hipSuccess
+             ? CHFFT_OK
+             : CHFFT_ERR_HIP;
 }
 
 int chfft_exec(chfft_plan plan, const void* in, void* out, int method, void* stream) {
@@ -1618,13 +1853,8 @@ int chfft_exec_host(chfft_plan plan, const void* in, void* out, int method) {
   if (hipMemcpy(plan->stage_in.p, in, ib, hipMemcpyHostToDevice) != hipSuccess) return CHFFT_ERR_HIP;
   if ((rc = chfft_exec(plan, plan->stage_in.p, plan->stage_out.p, method, nullptr))) return rc;
   if (hipMemcpy(out, plan->stage_out.p, ob, hipMemcpyDeviceToHost) != hipSuccess) return CHFFT_ERR_HIP;
-  if (plan->kind == K_MULTI) {
-    // a grid-wide barrier that timed out leaves its flag behind
-    unsigned sync[2] = {0, 0};
-    if (hipMemcpy(sync, plan->sync_dev.p, sizeof sync, hipMemcpyDeviceToHost) != hipSuccess) return CHFFT_ERR_HIP;
-    if (sync[1]) return CHFFT_ERR_HIP;
-  }
-  return CHFFT_OK;
+  // a grid-wide barrier of a multi-length launch that timed out leaves its flag behind
+  return read_barrier_flag(plan, nullptr);
 }
 
 int chfft_plan_describe(chfft_plan plan, char* buf, size_t buf_len) {
@@ -2056,6 +2286,14 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
       hipMemset(p->sync_dev.p, 0, 2 * sizeof(unsigned)) != hipSuccess)
     return CHFFT_ERR_HIP;
   p->nitems = (int)count;
+  // CHFFT_MULTI_BARRIER_TEST=1 (tests only): every grid-wide barrier waits
+  // for one more arrival than the grid makes, for 10 us, so that each wait
+  // gives up at once and the failure path runs with every wave exiting
+  if (const char* e = getenv("CHFFT_MULTI_BARRIER_TEST"))
+    if (strcmp(e, "1") == 0) {
+      p->barrier_ticks = 1000;
+      p->barrier_extra = 1;
+    }
   p->multi_lds = std::max<size_t>(lds, 16);
   p->multi_occ = multi_occupancy(f64, p->multi_lds);
   // every workgroup resident at once (the phases meet at grid-wide barriers)

@@ -42,7 +42,11 @@ typedef enum {
   CHFFT_ERR_INVALID_ARGUMENT = 2, /* null pointer, unknown enum value */
   CHFFT_ERR_UNSUPPORTED = 3,      /* method the transform type does not have */
   CHFFT_ERR_HIP = 4,              /* a HIP runtime call failed */
-  CHFFT_ERR_OUT_OF_MEMORY = 5
+  CHFFT_ERR_OUT_OF_MEMORY = 5,
+  /* a multi-length launch could not complete: one of its grid-wide barriers
+     timed out (its workgroups were not all resident), so the output is
+     incomplete; reported by chfft_plan_check and chfft_exec_host */
+  CHFFT_ERR_INCOMPLETE = 6
 } chfft_status;
 
 typedef enum { CHFFT_F32 = 0, CHFFT_F64 = 1 } chfft_precision;
@@ -70,11 +74,15 @@ int chfft_plan_c2c_1d(chfft_plan* plan, size_t len, size_t batch, int precision)
 
 /* RFft1D::new / setup (rfft1d.rs:73-113): len even (else
    CHFFT_ERR_INVALID_LENGTH). Forward: len reals -> len/2+1 complex per
-   transform; backward: the reverse. */
+   transform; backward: the reverse. Any even len: a half length above one
+   workgroup's LDS (FP32 16384, FP64 8192 points; 8192 / 4096 when it is not
+   7-smooth) runs its complex transform as a four-step over a plan-owned work
+   buffer, with the split as a pass of its own (also Dct1D and Mdct1D). */
 int chfft_plan_r2c_1d(chfft_plan* plan, size_t len, size_t batch, int precision);
 
 /* CFft2D::with_len / setup (cfft2d.rs:93-124): len_m rows of len_n complex
-   points, row-major, `batch` matrices. */
+   points, row-major, `batch` matrices. Columns longer than one workgroup's
+   LDS run as rows of the transposed matrices in a plan-owned work buffer. */
 int chfft_plan_c2c_2d(chfft_plan* plan, size_t len_m, size_t len_n, size_t batch, int precision);
 
 /* Dct1D::new / setup (dct1d.rs:53-86): dct_type CHFFT_DCT_II or
@@ -89,9 +97,21 @@ int chfft_plan_dct_1d(chfft_plan* plan, int dct_type, size_t len, int is_ortho, 
    b * frame_stride (frame_stride = len for independent frames, len/2 for a
    50 % overlapped signal; 0 means len), and writes len/2 coefficients per
    frame. Backward reads len/2 coefficients and writes len windowed samples
-   per frame. */
+   per frame. A plan whose frame_stride differs from len owns a buffer of
+   batch * len samples for chfft_exec_mdct_ola. */
 int chfft_plan_mdct_1d(chfft_plan* plan, size_t len, const double* window, size_t batch, size_t frame_stride,
                        int precision);
+
+/* Stream synthesis (no counterpart in the reference, whose Mdct1D::backward,
+   mdct1d.rs:244-291, returns windowed frames and leaves their overlap-add to
+   the caller): `method` CHFFT_BACKWARD or CHFFT_BACKWARDU of the plan's
+   `batch` coefficient frames, then the overlap-add of the windowed frames
+   at frame_stride, in frame order, into `out`, which receives
+   (batch - 1) * frame_stride + len samples. With a Princen-Bradley window,
+   hop len/2 and CHFFT_BACKWARD after CHFFT_FORWARD (or the U pair), the
+   samples from len/2 to (batch - 1) * frame_stride + len/2 reproduce the
+   analysed signal (TDAC). Device pointers, asynchronous on `stream`. */
+int chfft_exec_mdct_ola(chfft_plan plan, const void* in, void* out, int method, void* stream);
 
 /* Uneven batch (no counterpart type in the reference; each transform is the
    reference's CFft1D::forward/backward of its own length, cfft1d.rs:208-230):
@@ -106,6 +126,15 @@ int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t count, int
 
 /* Workgroup count of a multi-length plan's persistent launch (0 = default). */
 int chfft_plan_set_grid(chfft_plan plan, int grid);
+
+/* Waits for `stream` and reports whether the multi-length launches of this
+   plan since the last check completed: CHFFT_ERR_INCOMPLETE when one of
+   their grid-wide barriers timed out (the launch needs all its workgroups
+   resident at once; it is launched cooperatively where the stream is not
+   being captured, which guarantees that, and otherwise relies on the grid
+   being sized to the occupancy), else CHFFT_OK. Clears the record. Other
+   plan kinds: CHFFT_OK after the wait. */
+int chfft_plan_check(chfft_plan plan, void* stream);
 
 /* Autotuner (no counterpart in the reference): times the schedules this plan
    can run — for a multi-length plan the persistent launch's workgroup count,

@@ -88,3 +88,10 @@ def test_length_one_is_identity_without_gpu():
     x = np.array([3 + 4j])
     assert np.array_equal(f.forwardn(x), x)
     assert f.backward(np.zeros(0, complex)).shape == (0,)
+
+
+def test_new_entry_points_reject_null_plans():
+    L = _lib.lib()
+    assert L.chfft_plan_check(None, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.chfft_exec_mdct_ola(None, None, None, 4, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.chfft_status_string(_lib.ERR_INCOMPLETE).startswith(b"incomplete")

@@ -391,3 +391,17 @@ def test_cfft2d_all_scalings_vs_numpy(shape):
         assert rel(np.fft.fft(x, axis=1), F) > 0.1
     if n > 1:
         assert rel(np.fft.fft(x, axis=0), F) > 0.1
+
+
+@pytest.mark.parametrize("n", [4, 12, 64, 396])
+def test_brute_force_definitions_vs_golden(n):
+    # the oracle's long-double brute-force definitions (the reference's test
+    # formulas, mdct1d.rs:305-339 and dct1d.rs:423-434) against the golden
+    # float64 direct sums with integer-reduced angles
+    x = uniform_real(5000 + n, n)
+    w = np.sin(np.pi * (np.arange(n) + 0.5) / n)
+    assert rel(oracle.mdct_brute(x, w), G[f"mdct_{n}"]) < 1e-13
+    assert rel(oracle.imdct_brute(G[f"mdct_{n}"], w), G[f"imdct_{n}"]) < 1e-13
+    if n in (12,):
+        xd = uniform_real(4000 + n, n)
+        assert rel(oracle.dct2_brute(xd), G[f"dct2_{n}"]) < 1e-13
