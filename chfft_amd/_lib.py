@@ -15,7 +15,9 @@ COMPILE_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-OK, ERR_INVALID_LENGTH, ERR_INVALID_ARGUMENT, ERR_UNSUPPORTED, ERR_HIP, ERR_OOM, ERR_INCOMPLETE = range(7)
+OK, ERR_INVALID_LENGTH, ERR_INVALID_ARGUMENT, ERR_UNSUPPORTED, ERR_HIP, ERR_OOM, ERR_INCOMPLETE, \
+    ERR_SCRATCH_LIMIT = range(8)
+NOALLOC_SCRATCH_BYTES = 96 << 20
 F32, F64 = 0, 1
 METHODS = {
     "forward": 0, "forward0": 1, "forwardu": 2, "forwardn": 3,
@@ -69,6 +71,7 @@ def _signatures():
         "chfft_plan_autotune": ([vp, vp, vp, ci, vp], ci),
         "chfft_plan_destroy": ([vp], ci),
         "chfft_exec": ([vp, vp, vp, ci, vp], ci),
+        "chfft_exec_noalloc": ([vp, vp, vp, ci, vp], ci),
         "chfft_exec_host": ([vp, vp, vp, ci], ci),
         "chfft_exec_range": ([vp, vp, vp, ci, ci, ci, vp], ci),
         "chfft_plan_launch_count": ([vp, ci], ci),

@@ -1080,6 +1080,11 @@ struct C2CEngine {
 
 using namespace chfft;
 
+// Scratch of chfft_exec_noalloc: a file-scope device array that every GPU the
+// library runs on holds until the process ends. Its size is the entry
+// point's limit on a plan's four-step scratch plus work buffer.
+static __device__ __align__(256) unsigned char noalloc_scratch[CHFFT_NOALLOC_SCRATCH_BYTES];
+
 struct chfft_plan_s {
   Kind kind = K_C2C;
   bool f64 = false;
@@ -1115,13 +1120,42 @@ namespace {
 std::once_flag g_lds_once;
 hipError_t g_lds_err = hipSuccess;
 
+// Device address of noalloc_scratch on each device, looked up once per
+// device with that device current. In host code the array's name is only a
+// host-side handle: kernels receive these addresses, never the name.
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_addr[kMaxDevices] = {};
+
+int noalloc_base(void** base) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return CHFFT_ERR_HIP;
+  if (dev < 0 || dev >= kMaxDevices) return CHFFT_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_addr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return CHFFT_ERR_HIP;
+    }
+    g_noalloc_addr[dev] = p;
+  }
+  *base = g_noalloc_addr[dev];
+  return CHFFT_OK;
+}
+
+// Library set-up, run by every plan constructor: LDS limits of the kernels
+// (once per process) and the no-allocation scratch address of the current
+// device (once per device).
 int ensure_runtime() {
   std::call_once(g_lds_once, [] {
     g_lds_err = set_fused_lds_limits();
     hipError_t e2 = sfft_set_lds_limits();
     if (g_lds_err == hipSuccess) g_lds_err = e2;
   });
-  return g_lds_err == hipSuccess ? CHFFT_OK : CHFFT_ERR_HIP;
+  if (g_lds_err != hipSuccess) return CHFFT_ERR_HIP;
+  void* base = nullptr;
+  return noalloc_base(&base);
 }
 
 int check_prec(int precision) {
@@ -1252,6 +1286,7 @@ const char* chfft_status_string(int status) {
     case CHFFT_ERR_HIP: return "HIP runtime error";
     case CHFFT_ERR_OUT_OF_MEMORY: return "out of device memory";
     case CHFFT_ERR_INCOMPLETE: return "incomplete: a grid-wide barrier of the multi-length launch timed out";
+    case CHFFT_ERR_SCRATCH_LIMIT: return "the plan's scratch exceeds the no-allocation limit";
   }
   return "unknown status";
 }
@@ -1820,6 +1855,24 @@ int chfft_exec(chfft_plan plan, const void* in, void* out, int method, void* str
   int n = chfft_plan_launch_count(plan, method);
   if (n < 0) return -n;
   return chfft_exec_range(plan, in, out, method, 0, n, stream);
+}
+
+// chfft_exec with the plan's scratch and work buffers replaced by
+// noalloc_scratch: the scratch at its start, the work buffer at the next
+// 256-byte boundary. Every kernel writes each scratch and work element
+// before reading it within one call, so nothing needs clearing; calls that
+// overlap on one GPU share the array and race.
+int chfft_exec_noalloc(chfft_plan plan, const void* in, void* out, int method, void* stream) {
+  if (!plan || !in || !out) return CHFFT_ERR_INVALID_ARGUMENT;
+  const size_t sb = (plan->scratch.bytes + 255) / 256 * 256;
+  if (sb + plan->work.bytes > (size_t)CHFFT_NOALLOC_SCRATCH_BYTES) return CHFFT_ERR_SCRATCH_LIMIT;
+  int n = chfft_plan_launch_count(plan, method);
+  if (n < 0) return -n;
+  void* base = nullptr;
+  int rc = noalloc_base(&base);
+  if (rc) return rc;
+  return run_launches(plan, in, out, base, static_cast<char*>(base) + sb, method, 0, n,
+                      reinterpret_cast<hipStream_t>(stream));
 }
 
 int chfft_plan_launch_count(chfft_plan plan, int method) {

@@ -46,8 +46,14 @@ typedef enum {
   /* a multi-length launch could not complete: one of its grid-wide barriers
      timed out (its workgroups were not all resident), so the output is
      incomplete; reported by chfft_plan_check and chfft_exec_host */
-  CHFFT_ERR_INCOMPLETE = 6
+  CHFFT_ERR_INCOMPLETE = 6,
+  /* chfft_exec_noalloc: the plan needs more scratch than
+     CHFFT_NOALLOC_SCRATCH_BYTES */
+  CHFFT_ERR_SCRATCH_LIMIT = 7
 } chfft_status;
+
+/* Size of the device array behind chfft_exec_noalloc (96 MiB per GPU). */
+#define CHFFT_NOALLOC_SCRATCH_BYTES (96u << 20)
 
 typedef enum { CHFFT_F32 = 0, CHFFT_F64 = 1 } chfft_precision;
 
@@ -148,6 +154,18 @@ int chfft_plan_destroy(chfft_plan plan);
 
 /* Run `method` on device buffers, asynchronously on `stream`. */
 int chfft_exec(chfft_plan plan, const void* in, void* out, int method, void* stream);
+
+/* No counterpart in the reference: for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc). Same as chfft_exec, but the plan's four-step scratch
+   and work buffer are taken from a library-owned device array of
+   CHFFT_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until
+   the process ends; the call itself allocates and frees nothing. A plan
+   whose chfft_plan_scratch_bytes plus work buffer exceed the array gets
+   CHFFT_ERR_SCRATCH_LIMIT (e.g. the 43200 x 512 four-step fits in both
+   precisions; Bluestein 10007 x 512 in FP64 does not). All calls on one GPU
+   share the array: calls that overlap in time on one GPU race. */
+int chfft_exec_noalloc(chfft_plan plan, const void* in, void* out, int method, void* stream);
 
 /* Profiling hook (no counterpart in the reference): run only kernel launches
    [first, first + count) of chfft_exec's schedule for `method`, so that a
