@@ -1,20 +1,28 @@
 """Benchmark of the transform engine on one MI355X.
 
-Default (the driver's line): CFft1D forward on the mixed-radix workload
-N = 43200 (2^6 3^3 5^2), batch 512, complex N(0,1) entries from a seed. One
-step = one forward transform of the whole batch, inputs resident in HBM. The
-value is the FP32 form's throughput in GFLOP/s (nominal 5 N log2 N per
-transform); the FP64 plain method runs on the same inputs beside it, each with
-its rel-L2 against the FP64 oracle. `roofline` covers the dominant kernel of
-the FP32 schedule, timed with HIP events on the stream the kernels run on;
-`cpu_baseline` is the oracle (the C++ restatement of chfft's algorithm) timed
-on the host cores over a bounded sample of the same workload.
+One JSON line. Its top-level fields are the headline: CFft1D forward on the
+mixed-radix workload N = 43200 (2^6 3^3 5^2), batch 512, complex N(0,1)
+entries from a seed; one step = one forward transform of the whole batch,
+inputs resident in HBM. `value` is the FP32 form's throughput in GFLOP/s
+(nominal 5 N log2 N per transform); the FP64 plain method runs on the same
+inputs beside it, each with its rel-L2 against the FP64 oracle.
 
---suite prints one extra JSON line per configuration (all five workloads:
-c2c 4096 FP64/FP32, the headline, Bluestein primes, RFft/DCT/MDCT, the uneven
-multi-length batch replayed through a HIP graph plus CFft2D 2048x2048).
+`configs` holds every benchmark configuration (c1 plain FP64 C2C 4096 x 256,
+c2 the headline in both precisions, c3 Bluestein primes, c4 RFft/DCT/MDCT,
+c5 the uneven batch in one persistent launch and CFft2D 2048 x 2048, replayed
+through a HIP graph), one record per case with:
+  * `roofline`: algorithmic bytes of the whole transform (one read of the
+    input, one write of the output) over the summed duration of the
+    schedule's kernel launches, each bracketed by HIP events on the stream it
+    is launched on; `traffic` = HBM bytes per step counted by rocprofv3
+    (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy) where a counter run has
+    been recorded, else null; `launches` lists each launch's average ms;
+  * `cpu_baseline`: the oracle (a C++ restatement of chfft's algorithm,
+    built with -march=native on this host where g++ is present) timed on a
+    bounded sample of the same workload, plans built before the clock, on
+    min(cores, 16) threads and on 1 core.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--suite] [--no-cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only ID ...] [--copy-bytes B]
 """
 import argparse
 import ctypes
@@ -22,6 +30,8 @@ import json
 import math
 import os
 import sys
+import tempfile
+import time
 
 import numpy as np
 
@@ -32,6 +42,8 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "FFT GFLOP/s (5Nlog2N) + % HBM roofline: FP32-batched vs FP64-plain @ rel-L2"
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
 ROTATE_BYTES = 1 << 30  # buffer pools rotated so that reads come from HBM, not the 256 MB MALL
+CPU_REP_SECONDS = 0.6   # CPU baseline sample sized for about this long per pass (3 passes, best)
+CONFIG_IDS = ("c1", "c2", "c3", "c4", "c5")
 
 
 def parse():
@@ -39,8 +51,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--suite", action="store_true", help="also print one line per configuration")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--only", nargs="*", default=None, choices=CONFIG_IDS,
+                    help="configurations to run (default: all; c2 is always the headline)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only time a plain device-to-device copy of this many bytes (the reference kernel for "
                          "the HBM traffic counters and the achievable copy bandwidth)")
@@ -66,8 +79,8 @@ def copy_calibration(torch, nbytes, reps=10):
 
 
 class Bufs:
-    """A pool of input/output buffer pairs (rotated when one pair is small
-    enough to stay in the MALL)."""
+    """A pool of input/output buffer pairs, rotated when one pair is small
+    enough to stay in the 256 MB memory-side cache."""
 
     def __init__(self, torch, in_bytes, out_bytes, fill, rotate=True):
         pair = in_bytes + out_bytes
@@ -85,11 +98,23 @@ class Bufs:
         return self.ins[j].data_ptr(), self.outs[j].data_ptr()
 
 
+class Fixed:
+    """Explicit (input, output) tensor pairs, rotated."""
+
+    def __init__(self, pairs):
+        self.pairs = pairs
+        self.k = len(pairs)
+
+    def pair(self, i):
+        a, b = self.pairs[i % self.k]
+        return a.data_ptr(), b.data_ptr()
+
+
 def time_plan(torch, plan, bufs, steps, warmup, method="forward", blocks=1):
-    """ms per step over `steps` steps bracketed by events; with blocks > 1
-    the median of that many such blocks (the per-configuration lines, whose
-    microsecond steps are sensitive to one-off host stalls)."""
-    st = torch.cuda.current_stream().cuda_stream
+    """ms per step over `steps` steps bracketed by events on the launch
+    stream; with blocks > 1 the median of that many such blocks."""
+    s = torch.cuda.current_stream()
+    st = s.cuda_stream
     for i in range(warmup):
         a, b = bufs.pair(i)
         plan.exec_ptr(a, b, method, st)
@@ -98,11 +123,11 @@ def time_plan(torch, plan, bufs, steps, warmup, method="forward", blocks=1):
     for _ in range(blocks):
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(torch.cuda.current_stream())
+        e0.record(s)
         for i in range(steps):
             a, b = bufs.pair(i)
             plan.exec_ptr(a, b, method, st)
-        e1.record(torch.cuda.current_stream())
+        e1.record(s)
         torch.cuda.synchronize()
         res.append(e0.elapsed_time(e1) / steps)
     return float(np.median(res))
@@ -131,15 +156,9 @@ def per_kernel_ms(torch, plan, bufs, steps, method="forward"):
 
 
 def rel_l2(a, b):
-    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b))
-
-
-def cplx_fill(torch, seed):
-    def fill(a, i):
-        g = torch.Generator(device="cuda")
-        g.manual_seed(seed + i)
-        a.normal_(generator=g)
-    return fill
+    a = np.asarray(a, dtype=np.complex128)
+    b = np.asarray(b, dtype=np.complex128)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
 def traffic_for(key):
@@ -149,9 +168,46 @@ def traffic_for(key):
     return None
 
 
-def c2c_case(torch, n, batch, f64, steps, warmup, seed, check=8, uniform=False, blocks=1):
-    """Throughput, per-kernel times and rel-L2 (first `check` transforms vs
-    the FP64 oracle) of CFft1D forward on [batch, n]."""
+def roofline(alg_bytes, kms, key):
+    """Whole-transform roofline of one step: algorithmic bytes over the
+    summed launch durations (ms)."""
+    t = sum(kms)
+    ach = alg_bytes / (t * 1e6)
+    tr = traffic_for(key)
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tr, "algorithmic_bytes": alg_bytes,
+            "kernel": f"whole schedule ({len(kms)} launch{'es' if len(kms) > 1 else ''} per step)",
+            "launches_ms": [round(v, 5) for v in kms], "traffic_key": key}
+
+
+def cpu_rate(run, flop_per_unit, full_units, threads, label, note):
+    """Times run(units, threads) -> seconds per pass on a sample of `units`
+    grown until one pass takes about CPU_REP_SECONDS (capped at the full
+    workload); GFLOP/s over the sample."""
+    units = max(1, min(full_units, 4))
+    while True:
+        sec = run(units, threads)
+        if sec >= CPU_REP_SECONDS * 0.5 or units >= full_units:
+            break
+        units = min(full_units, max(units + 1, int(units * CPU_REP_SECONDS / max(sec, 1e-4))))
+    return {"value": round(flop_per_unit * units / sec / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
+            "kind": "port", "sample": f"{label}: {units} of {full_units}, best of 3 passes, plans built before "
+                                      f"the clock; oracle {note}"}
+
+
+def cpu_both(run, flop_per_unit, full_units, label, note):
+    threads = min(os.cpu_count() or 1, 16)
+    d = cpu_rate(run, flop_per_unit, full_units, threads, label, note)
+    one = cpu_rate(run, flop_per_unit, max(1, full_units), 1, label, note)
+    d["value_1core"] = one["value"]
+    d["sample_1core"] = one["sample"]
+    return d
+
+
+# ---------------------------------------------------------------- cases
+def c2c_case(torch, n, batch, f64, args, seed, uniform=False, blocks=1, check=8, cpu_note=None, traffic_key=None):
+    """CFft1D forward on [batch, n]: throughput, per-launch times, rel-L2 of
+    `check` transforms against the FP64 oracle, roofline, CPU baseline."""
     import oracle
     from chfft_amd import Plan
     p = Plan("c2c", f64, batch, n)
@@ -165,107 +221,111 @@ def c2c_case(torch, n, batch, f64, steps, warmup, seed, check=8, uniform=False, 
         else:
             a.normal_(generator=g)
     bufs = Bufs(torch, ib, ob, fill)
-    ms = time_plan(torch, p, bufs, steps, warmup, blocks=blocks)
-    kms = per_kernel_ms(torch, p, bufs, max(3, steps // 2))
-    cdt = np.complex128 if f64 else np.complex64
+    ms = time_plan(torch, p, bufs, args.steps, args.warmup, blocks=blocks)
+    kms = per_kernel_ms(torch, p, bufs, max(3, args.steps // 2))
+    tdt = torch.complex128 if f64 else torch.complex64
     k = min(check, batch)
-    fl = k * n * (4 if f64 else 2)  # float32 words of k complex rows
-    x = bufs.ins[0][:fl].view(torch.complex128 if f64 else torch.complex64).cpu().numpy().reshape(k, n)
+    fl = k * n * (4 if f64 else 2)
+    x = bufs.ins[0][:fl].view(tdt).cpu().numpy().reshape(k, n)
     p.exec_ptr(bufs.ins[0].data_ptr(), bufs.outs[0].data_ptr(), "forward", torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
-    y = bufs.outs[0][:fl].view(torch.complex128 if f64 else torch.complex64).cpu().numpy().reshape(k, n)
-    ref = oracle.c2c(x.astype(np.complex128), "forward") if n <= 70000 else np.fft.fft(x.astype(np.complex128))
-    err = rel_l2(y.astype(np.complex128), ref)
-    flop = 5.0 * n * math.log2(n) * batch
+    y = bufs.outs[0][:fl].view(tdt).cpu().numpy().reshape(k, n)
+    xr = x.astype(np.complex128)
+    ref = oracle.c2c(xr, "forward") if n <= 70000 else np.fft.fft(xr)
+    flop = 5.0 * n * math.log2(n)
     alg = (32.0 if f64 else 16.0) * n * batch
-    return {"n": n, "batch": batch, "dtype": "f64" if f64 else "f32", "ms": round(ms, 5),
-            "gflops": round(flop / (ms * 1e6), 2), "hbm_frac": round(alg / (ms * 1e6) / HBM_PEAK_GBS, 4),
-            "kernel_ms": [round(v, 5) for v in kms], "rel_l2": err, "buffers_rotated": bufs.k,
-            "schedule": p.describe().split("inverse:")[0].replace("forward:", "").strip()}, p, bufs, cdt
+    rec = {"n": n, "batch": batch, "dtype": "f64" if f64 else "f32", "ms_per_step": round(ms, 5),
+           "gflops": round(flop * batch / (ms * 1e6), 2), "hbm_frac_step": round(alg / (ms * 1e6) / HBM_PEAK_GBS, 4),
+           "rel_l2": rel_l2(y, ref), "buffers_rotated": bufs.k,
+           "roofline": roofline(alg, kms, traffic_key or f"c2c_{n}x{batch}_{'f64' if f64 else 'f32'}")}
+    if cpu_note is not None:
+        big = bufs.ins[0].view(tdt)[:batch * n].cpu().numpy().reshape(batch, n)
+
+        def run(units, threads):
+            return oracle.time_c2c(big[:units], "forward", threads=threads, reps=3)
+        rec["cpu_baseline"] = cpu_both(run, flop, batch, f"CFft1D<{'f64' if f64 else 'f32'}>::forward, N={n}, "
+                                                         f"transforms", cpu_note)
+    return rec, p, bufs
 
 
-def headline(torch, args):
-    import oracle
+def headline_and_c2(torch, args, cpu_note):
     N, BATCH, SEED = 43200, 512, 0x5EED0002
-    r32, p32, b32, _ = c2c_case(torch, N, BATCH, False, args.steps, args.warmup, SEED)
-    r64, p64, b64, _ = c2c_case(torch, N, BATCH, True, args.steps, args.warmup, SEED)
-    k32 = r32["kernel_ms"]
-    dom = int(np.argmax(k32))
-    # the split four-step runs in equal chunks of transforms, two launches
-    # (columns, rows) per chunk; each launch reads and writes its chunk once:
-    # algorithmic bytes per launch = 16 N per transform of the chunk (FP32)
-    chunks = max(1, len(k32) // 2)
-    bytes_launch = 16.0 * N * BATCH / chunks
-    achieved = bytes_launch / (k32[dom] * 1e6)
-    cpu = None
-    if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        xs = b32.ins[0][:64 * N * 2].view(torch.complex64).cpu().numpy().reshape(64, N)
-        sec = oracle.time_c2c(xs, "forward", threads=threads, reps=3)
-        cpu = {"value": round(5.0 * N * math.log2(N) * 64 / sec / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
-               "kind": "port",
-               "sample": f"oracle (C++ restatement of chfft's mixed-radix CFft1D<f32>::forward) on 64 of the "
-                         f"512 transforms of N={N} on {threads} threads, best of 3 passes, plan built before timing"}
-    return {
+    r32, p32, b32 = c2c_case(torch, N, BATCH, False, args, SEED, cpu_note=cpu_note)
+    r64, _, _ = c2c_case(torch, N, BATCH, True, args, SEED, cpu_note=cpu_note)
+    sched = p32.describe().split("inverse:")[0].replace("forward:", "").strip()
+    kms = r32["roofline"]["launches_ms"]
+    dom = int(np.argmax(kms))
+    # the dominant launch on its own: it moves one read and one write of its
+    # chunk's data, half of each transform's passes
+    chunks = max(1, len(kms) // 2)
+    launch_bytes = 16.0 * N * BATCH / chunks
+    head = {
         "metric": METRIC,
         "value": r32["gflops"],
         "unit": "GFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r32["ms"],
+        "ms_per_step": r32["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: complex entries i.i.d. N(0,1), torch generator seed 0x5EED0002",
-        "config": {"workload": "CFft1D forward, mixed radix N=43200 (2^6 3^3 5^2), batch 512",
-                   "n": N, "batch": BATCH, "schedule": r32["schedule"]},
-        "hbm_frac_whole_transform": r32["hbm_frac"],
+        "config": {"workload": "CFft1D forward, mixed radix N=43200 (2^6 3^3 5^2), batch 512", "n": N,
+                   "batch": BATCH, "schedule": sched},
         "rel_l2_f32_vs_f64_oracle": r32["rel_l2"],
-        "fp64_plain": {"value": r64["gflops"], "unit": "GFLOP/s", "ms_per_step": r64["ms"],
-                       "hbm_frac_whole_transform": r64["hbm_frac"], "rel_l2_vs_f64_oracle": r64["rel_l2"],
-                       "kernel_ms": r64["kernel_ms"]},
-        "speedup_f32_over_f64": round(r64["ms"] / r32["ms"], 3),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for("c2c_43200x512_f32_launch%d" % dom),
-                     "kernel": "launch %d of %d (%s)" % (dom, len(k32), r32["schedule"].splitlines()[dom].strip()),
-                     "kernel_ms": k32, "bytes_per_launch": bytes_launch},
-        "cpu_baseline": cpu,
+        "fp64_plain": {"value": r64["gflops"], "unit": "GFLOP/s", "ms_per_step": r64["ms_per_step"],
+                       "rel_l2_vs_f64_oracle": r64["rel_l2"], "roofline_frac": r64["roofline"]["frac"],
+                       "launches_ms": r64["roofline"]["launches_ms"]},
+        "speedup_f32_over_f64": round(r64["ms_per_step"] / r32["ms_per_step"], 3),
+        "roofline": dict(r32["roofline"], dominant_launch={
+            "index": dom, "ms": kms[dom], "bytes": launch_bytes,
+            "achieved": round(launch_bytes / (kms[dom] * 1e6), 1),
+            "what": sched.splitlines()[dom].strip() if dom < len(sched.splitlines()) else ""}),
+        "cpu_baseline": r32.get("cpu_baseline"),
     }
+    cfg = {"id": "c2", "workload": "CFft1D forward, N=43200 (2^6 3^3 5^2) x 512, complex N(0,1); FP32 form and "
+                                   "FP64 plain method", "cases": [r32, r64]}
+    return head, cfg
 
 
-def suite(torch, args):
-    """One line per configuration (printed after the headline)."""
-    import oracle
-    from chfft_amd import MultiFFT, Plan
-    from chfft_amd.workloads import uneven_lengths
-    lines = []
-    st = torch.cuda.current_stream().cuda_stream
-
-    def line(cfg, recs, extra=None):
-        d = {"config": cfg, "results": recs}
-        if extra:
-            d.update(extra)
-        lines.append(d)
-        print(json.dumps(d), flush=True)
-
-    # 1: plain method, CFft1D<f64> N=4096 forward then backward, batch 256 (+ FP32 form)
+def cfg_c1(torch, args, cpu_note):
     recs = []
     for f64 in (True, False):
-        r, p, b, _ = c2c_case(torch, 4096, 256, f64, args.steps, args.warmup, 0x5EED0001, uniform=True, blocks=5)
-        msb = time_plan(torch, p, b, args.steps, args.warmup, "backward", blocks=5)
-        r["ms_backward"] = round(msb, 5)
+        r, p, b = c2c_case(torch, 4096, 256, f64, args, 0x5EED0001, uniform=True, blocks=5, cpu_note=cpu_note)
+        r["ms_backward"] = round(time_plan(torch, p, b, args.steps, args.warmup, "backward", blocks=5), 5)
         recs.append(r)
-    line("c1: CFft1D N=4096 batch 256, forward then backward", recs)
-    # 3: Bluestein primes
+    return {"id": "c1", "workload": "CFft1D N=4096 x 256, uniform[-1,1); forward (timed) then backward (1/N); "
+                                    "FP64 plain method and FP32 form", "cases": recs}
+
+
+def cfg_c3(torch, args, cpu_note):
     recs = []
     for n, batch in ((4093, 2048), (10007, 512), (65537, 64)):
         for f64 in (False, True):
-            recs.append(c2c_case(torch, n, batch, f64, args.steps, args.warmup, 0x5EED0003, uniform=True,
-                                 blocks=5)[0])
-    line("c3: Bluestein chirp-z, N in {4093, 10007, 65537}", recs)
-    # 4: RFft1D 8192 x 4096, Dct1D II/III 1024 x 16384, Mdct1D 2048 hop 1024 over 2^24 samples
+            r, p, _ = c2c_case(torch, n, batch, f64, args, 0x5EED0003, uniform=True, blocks=5,
+                               cpu_note=cpu_note if not f64 else None)
+            d = p.describe()
+            r["padded_M"] = int(d.split(" M L=")[1].split()[0]) if " M L=" in d else bluestein_m(d)
+            recs.append(r)
+    return {"id": "c3", "workload": "Bluestein chirp-z, N in {4093 x 2048, 10007 x 512, 65537 x 64}, "
+                                    "uniform[-1,1); chirp index n^2 mod 2N in exact integers; padded_M is the "
+                                    "inner FFT length each case ran", "cases": recs}
+
+
+def bluestein_m(desc):
+    """Padded length of a four-step Bluestein schedule: product of its column
+    and row lengths (from the plan description)."""
+    import re
+    ls = [int(v) for v in re.findall(r"L=(\d+)", desc.split("inverse:")[0])]
+    return ls[0] * ls[1] if len(ls) >= 2 else None
+
+
+def cfg_c4(torch, args, cpu_note):
+    import oracle
+    from chfft_amd import Plan
+    st = torch.cuda.current_stream().cuda_stream
     recs = []
     for kind, n, batch in (("r2c", 8192, 4096), ("dct2", 1024, 16384), ("dct3", 1024, 16384)):
         for f64 in (False, True):
@@ -276,12 +336,20 @@ def suite(torch, args):
                 p = Plan("dct", f64, batch, 2 if kind == "dct2" else 3, n, False)
                 alg = 8 * n * batch * (2 if f64 else 1)
             ib, ob = p.io_bytes("forward")
-            b = Bufs(torch, ib, ob, cplx_fill(torch, 0x5EED0004))
+
+            def fill(a, i, seed=0x5EED0004):
+                g = torch.Generator(device="cuda")
+                g.manual_seed(seed + i)
+                if kind == "r2c":
+                    a.normal_(generator=g)
+                else:
+                    a.uniform_(-1, 1, generator=g)
+            b = Bufs(torch, ib, ob, fill)
             ms = time_plan(torch, p, b, args.steps, args.warmup, blocks=5)
-            # accuracy on 8 rows
+            kms = per_kernel_ms(torch, p, b, max(3, args.steps // 2))
+            rdt = torch.float64 if f64 else torch.float32
             rows = 8
-            xr = b.ins[0][:rows * n * (2 if f64 else 1)].view(torch.float64 if f64 else torch.float32)
-            xr = xr.cpu().numpy()[:rows * n].reshape(rows, n).astype(np.float64)
+            xr = b.ins[0].view(rdt)[:rows * n].cpu().numpy().reshape(rows, n).astype(np.float64)
             p.exec_ptr(b.ins[0].data_ptr(), b.outs[0].data_ptr(), "forward", st)
             torch.cuda.synchronize()
             if kind == "r2c":
@@ -290,12 +358,21 @@ def suite(torch, args):
                 got = got.cpu().numpy().reshape(rows, n // 2 + 1)
             else:
                 ref = oracle.dct(xr, 2 if kind == "dct2" else 3)
-                got = b.outs[0].view(torch.float64 if f64 else torch.float32)[:rows * n].cpu().numpy().reshape(rows, n)
-            recs.append({"kind": kind, "n": n, "batch": batch, "dtype": "f64" if f64 else "f32", "ms": round(ms, 5),
-                         "gflops": round(2.5 * n * math.log2(n) * batch / (ms * 1e6), 2),
-                         "hbm_frac": round(alg / (ms * 1e6) / HBM_PEAK_GBS, 4),
-                         "rel_l2": rel_l2(got.astype(np.complex128), ref.astype(np.complex128)),
-                         "schedule": p.describe().split("inverse:")[0].replace("forward:", "").strip()})
+                got = b.outs[0].view(rdt)[:rows * n].cpu().numpy().reshape(rows, n)
+            flop = 2.5 * n * math.log2(n)
+            rec = {"kind": kind, "n": n, "batch": batch, "dtype": "f64" if f64 else "f32",
+                   "ms_per_step": round(ms, 5), "gflops": round(flop * batch / (ms * 1e6), 2),
+                   "rel_l2": rel_l2(got, ref), "buffers_rotated": b.k,
+                   "roofline": roofline(alg, kms, f"{kind}_{n}x{batch}_{'f64' if f64 else 'f32'}")}
+            if cpu_note is not None and not f64:
+                full = b.ins[0].view(rdt)[:batch * n].cpu().numpy().reshape(batch, n)
+
+                def run(units, threads, full=full, kind=kind, n=n):
+                    return oracle.time_op("rfft" if kind == "r2c" else kind, full[:units], n, threads=threads)
+                label = {"r2c": "RFft1D<f32>::forward", "dct2": "Dct1D<f32> type II forward",
+                         "dct3": "Dct1D<f32> type III forward"}[kind] + f", N={n}, rows"
+                rec["cpu_baseline"] = cpu_both(run, flop, batch, label, cpu_note)
+            recs.append(rec)
     # MDCT over a 2^24-sample signal: 8 sinusoids + N(0, 0.01) noise, hop 1024
     n, hop, S = 2048, 1024, 1 << 24
     frames = (S - n) // hop + 1
@@ -306,36 +383,44 @@ def suite(torch, args):
     for f64 in (False, True):
         win = oracle.window(n, "sine")
         p = Plan("mdct", f64, frames, n, win, hop)
-        xs = torch.from_numpy(sig.astype(np.float64 if f64 else np.float32)).cuda()
+        sd = sig.astype(np.float64 if f64 else np.float32)
+        xs = torch.from_numpy(sd).cuda()
         ys = torch.empty(frames * n // 2, dtype=xs.dtype, device="cuda")
         # copies of the signal rotated like the other cases, so that the
-        # signal is read from HBM rather than the 256 MB MALL
+        # signal is read from HBM rather than the 256 MB cache
         kr = max(1, min(16, ROTATE_BYTES // (xs.numel() * xs.element_size() + ys.numel() * ys.element_size())))
-        pool = [(xs, ys)] + [(xs.clone(), torch.empty_like(ys)) for _ in range(kr - 1)]
-
-        class Rot:
-            k = kr
-
-            def pair(self, i):
-                a, b = pool[i % kr]
-                return a.data_ptr(), b.data_ptr()
-        ms = time_plan(torch, p, Rot(), args.steps, args.warmup, blocks=5)
+        bufs = Fixed([(xs, ys)] + [(xs.clone(), torch.empty_like(ys)) for _ in range(kr - 1)])
+        ms = time_plan(torch, p, bufs, args.steps, args.warmup, blocks=5)
+        kms = per_kernel_ms(torch, p, bufs, max(3, args.steps // 2))
         p.exec_ptr(xs.data_ptr(), ys.data_ptr(), "forward", st)
         torch.cuda.synchronize()
         chk = 64
         ref = oracle.mdct(sig[:(chk - 1) * hop + n].astype(np.float64), win, "forward", n=n, hop=hop, frames=chk)
         got = ys[:chk * n // 2].cpu().numpy().reshape(chk, n // 2)
         alg = (S + frames * n // 2) * (8 if f64 else 4)
-        del pool
-        recs.append({"kind": "mdct", "n": n, "hop": hop, "frames": frames, "dtype": "f64" if f64 else "f32",
-                     "ms": round(ms, 5), "gflops": round(2.5 * n * math.log2(n) * frames / (ms * 1e6), 2),
-                     "hbm_frac": round(alg / (ms * 1e6) / HBM_PEAK_GBS, 4), "rel_l2": rel_l2(got, ref),
-                     "buffers_rotated": kr,
-                     "schedule": p.describe().split("inverse:")[0].replace("forward:", "").strip()})
-    line("c4: RFft1D 8192 x 4096, Dct1D II/III 1024 x 16384, Mdct1D 2048 hop 1024 over 2^24 samples", recs)
+        flop = 2.5 * n * math.log2(n)
+        rec = {"kind": "mdct", "n": n, "hop": hop, "frames": frames, "dtype": "f64" if f64 else "f32",
+               "ms_per_step": round(ms, 5), "gflops": round(flop * frames / (ms * 1e6), 2),
+               "rel_l2": rel_l2(got, ref), "buffers_rotated": kr,
+               "roofline": roofline(alg, kms, f"mdct_{n}x{frames}_{'f64' if f64 else 'f32'}")}
+        if cpu_note is not None and not f64:
+            w32 = oracle.window(n, "sine", np.float32)
 
-    # 5: uneven batch in one persistent launch, replayed through a HIP graph,
-    # grid autotuned; plus CFft2D 2048 x 2048, all FP32
+            def run(units, threads):
+                return oracle.time_op("mdct", sd, n, threads=threads, batch=units, hop=hop, window=w32)
+            rec["cpu_baseline"] = cpu_both(run, flop, frames, "Mdct1D<f32>::forward (sine), N=2048 hop 1024, "
+                                                               "frames", cpu_note)
+        del bufs
+        recs.append(rec)
+    return {"id": "c4", "workload": "RFft1D 8192 x 4096 (real N(0,1)); Dct1D II and III 1024 x 16384 "
+                                    "(uniform[-1,1)); Mdct1D 2048 (sine window, hop 1024) over a 2^24-sample "
+                                    "signal of 8 sinusoids + N(0, 0.01) noise, 16383 frames", "cases": recs}
+
+
+def cfg_c5(torch, args, cpu_note):
+    import oracle
+    from chfft_amd import MultiFFT, Plan
+    from chfft_amd.workloads import uneven_lengths
     lens = uneven_lengths()
     fft = MultiFFT(lens, np.float32)
     x = torch.empty(fft.total, dtype=torch.complex64, device="cuda")
@@ -363,6 +448,7 @@ def suite(torch, args):
     for _ in range(2):
         graph.replay()
     torch.cuda.synchronize()
+    fft.plan.check()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     reps = max(3, args.steps // 4)
@@ -371,28 +457,16 @@ def suite(torch, args):
         graph.replay()
     e1.record()
     torch.cuda.synchronize()
+    fft.plan.check()
     ms_graph = e0.elapsed_time(e1) / reps
-    # the two parts separately (eager)
-
-    class P1:
-        k = 1
-
-        def pair(self, i):
-            return x.data_ptr(), y.data_ptr()
-
-    # the 2-D transform alone: 67 MB per pair, rotated over 1 GB of pairs so
-    # that it reads HBM rather than the MALL
+    one = Fixed([(x, y)])
+    ms_multi = time_plan(torch, fft.plan, one, reps, 1)
+    kms_multi = per_kernel_ms(torch, fft.plan, one, 3)
+    fft.plan.check()
     k2 = max(1, ROTATE_BYTES // (2 * x2.numel() * x2.element_size()))
-    pool2 = [(x2, y2)] + [(x2.clone(), torch.empty_like(y2)) for _ in range(k2 - 1)]
-
-    class P2:
-        k = k2
-
-        def pair(self, i):
-            a, b = pool2[i % k2]
-            return a.data_ptr(), b.data_ptr()
-    ms_multi = time_plan(torch, fft.plan, P1(), max(3, args.steps // 4), 1)
-    ms_2d = time_plan(torch, p2, P2(), args.steps, args.warmup, blocks=5)
+    pool2 = Fixed([(x2, y2)] + [(x2.clone(), torch.empty_like(y2)) for _ in range(k2 - 1)])
+    ms_2d = time_plan(torch, p2, pool2, args.steps, args.warmup, blocks=5)
+    kms_2d = per_kernel_ms(torch, p2, pool2, max(3, args.steps // 2))
     del pool2
     # rel-L2 per octave bucket on a sample of transforms (numpy FP64 reference)
     xs = x.cpu().numpy()
@@ -401,29 +475,45 @@ def suite(torch, args):
     for i in range(0, len(lens), 7):
         o0, o1 = fft.offsets[i], fft.offsets[i + 1]
         n = lens[i]
-        e = rel_l2(ys[o0:o1].astype(np.complex128), np.fft.fft(xs[o0:o1].astype(np.complex128)))
-        b = int(math.log2(n))
-        buckets.setdefault(b, []).append(e)
+        e = rel_l2(ys[o0:o1], np.fft.fft(xs[o0:o1].astype(np.complex128)))
+        buckets.setdefault(int(math.log2(n)), []).append(e)
     bucket_err = {f"2^{b}..2^{b + 1}": {"max": max(v), "count": len(v), "budget": 1e-6 * (b + 1)}
                   for b, v in sorted(buckets.items())}
     x2h = x2.cpu().numpy().reshape(2048, 2048).astype(np.complex128)
     e2d = rel_l2(y2.cpu().numpy().reshape(2048, 2048), np.fft.fft2(x2h))
     flop_multi = sum(5.0 * n * math.log2(n) for n in lens if n > 1)
-    alg_multi = 16.0 * fft.total
     flop_2d = 5.0 * 2048 * 2048 * math.log2(2048 * 2048)
-    line("c5: 10,000 uneven FP32 transforms (log-uniform 16..2^20, 50/50 2*3*5-smooth/prime, <= 2^28 points) "
-         "in one persistent launch + CFft2D 2048x2048, replayed through one HIP graph",
-         [{"part": "multi", "transforms": len(lens), "points": fft.total, "ms": round(ms_multi, 4),
-           "gflops": round(flop_multi / (ms_multi * 1e6), 2),
-           "hbm_frac": round(alg_multi / (ms_multi * 1e6) / HBM_PEAK_GBS, 4), "autotune_choice": choice,
-           "schedule": fft.plan.describe().strip(), "rel_l2_per_bucket": bucket_err},
-          {"part": "cfft2d", "m": 2048, "n": 2048, "ms": round(ms_2d, 4),
-           "gflops": round(flop_2d / (ms_2d * 1e6), 2),
-           "hbm_frac": round(16.0 * 2048 * 2048 / (ms_2d * 1e6) / HBM_PEAK_GBS, 4), "rel_l2": e2d,
-           "schedule": p2.describe().split("inverse:")[0].replace("forward:", "").strip()},
-          {"part": "graph (multi + 2-D)", "ms": round(ms_graph, 4),
-           "gflops": round((flop_multi + flop_2d) / (ms_graph * 1e6), 2)}])
-    return lines
+    rm = {"part": "multi", "transforms": len(lens), "points": fft.total, "dtype": "f32",
+          "ms_per_step": round(ms_multi, 4), "gflops": round(flop_multi / (ms_multi * 1e6), 2),
+          "autotune_choice": choice, "rel_l2_per_bucket": bucket_err,
+          "schedule": fft.plan.describe().strip().splitlines()[0],
+          "roofline": roofline(16.0 * fft.total, kms_multi, "multi_uneven10000_f32")}
+    r2 = {"part": "cfft2d", "m": 2048, "n": 2048, "dtype": "f32", "ms_per_step": round(ms_2d, 4),
+          "gflops": round(flop_2d / (ms_2d * 1e6), 2), "rel_l2": e2d,
+          "roofline": roofline(16.0 * 2048 * 2048, kms_2d, "c2c2d_2048x2048_f32")}
+    rg = {"part": "graph (multi + 2-D)", "ms_per_step": round(ms_graph, 4),
+          "gflops": round((flop_multi + flop_2d) / (ms_graph * 1e6), 2)}
+    if cpu_note is not None:
+        # a sample of the uneven batch: every 50th transform (the same size mix)
+        idx = list(range(0, len(lens), 50))
+        sl = [lens[i] for i in idx]
+        sx = np.concatenate([xs[fft.offsets[i]:fft.offsets[i + 1]] for i in idx])
+        sflop = sum(5.0 * n * math.log2(n) for n in sl if n > 1)
+
+        def run_m(units, threads):
+            k = min(units, len(sl))
+            return oracle.time_lens(sx[:sum(sl[:k])], sl[:k], threads=threads, reps=3)
+        d = cpu_both(run_m, sflop / len(sl), len(sl), "CFft1D<f32>::forward over every 50th transform of the "
+                                                      "uneven batch (one plan per transform)", cpu_note)
+        rm["cpu_baseline"] = d
+        x2s = x2.cpu().numpy().reshape(1, 2048, 2048)
+
+        def run_2d(units, threads):
+            return oracle.time_op("c2c2d", x2s, 2048, m=2048, threads=threads)
+        r2["cpu_baseline"] = cpu_both(run_2d, flop_2d, 1, "CFft2D<f32>::forward 2048 x 2048, matrices", cpu_note)
+    return {"id": "c5", "workload": "10,000 uneven FP32 transforms (log-uniform 16..2^20, 50/50 2*3*5-smooth / "
+                                    "prime, <= 2^28 points, N(0,1)) in one persistent launch, plus CFft2D 2048 x "
+                                    "2048; both replayed through one HIP graph", "cases": [rm, r2, rg]}
 
 
 def main():
@@ -438,9 +528,27 @@ def main():
     if args.copy_bytes:
         print(json.dumps(copy_calibration(torch, args.copy_bytes)), flush=True)
         return
-    print(json.dumps(headline(torch, args)), flush=True)
-    if args.suite:
-        suite(torch, args)
+    t0 = time.time()
+    cpu_note = None
+    tmp = None
+    if not args.no_cpu:
+        import oracle
+        tmp = tempfile.TemporaryDirectory(prefix="chfft_oracle_")
+        cpu_note = oracle.use_native(tmp.name)
+    only = set(args.only) if args.only else set(CONFIG_IDS)
+    head, c2 = headline_and_c2(torch, args, cpu_note)
+    configs = []
+    for cid, fn in (("c1", cfg_c1), ("c2", None), ("c3", cfg_c3), ("c4", cfg_c4), ("c5", cfg_c5)):
+        if cid not in only:
+            continue
+        print(f"bench: {cid} ({time.time() - t0:.0f} s)", file=sys.stderr, flush=True)
+        configs.append(c2 if cid == "c2" else fn(torch, args, cpu_note))
+        torch.cuda.empty_cache()
+    head["configs"] = configs
+    head["bench_seconds"] = round(time.time() - t0, 1)
+    print(json.dumps(head), flush=True)
+    if tmp is not None:
+        tmp.cleanup()
 
 
 if __name__ == "__main__":

@@ -31,12 +31,38 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+_NATIVE_NOTE = None
+
+
+def use_native(tmpdir):
+    """Build the oracle for this host (-march=native) into tmpdir and use it
+    from now on (the CPU baseline). Returns a note saying which build runs."""
+    global _LIB, _NATIVE_NOTE
+    out = os.path.join(tmpdir, "liboracle_native.so")
+    try:
+        subprocess.run(["make", "-s", "-C", _HERE, "native", f"OUT={out}"], check=True, timeout=300,
+                       capture_output=True)
+        _LIB = None
+        _load(out)
+        _NATIVE_NOTE = "built with g++ -O3 -march=native on this host"
+    except Exception as e:  # noqa: BLE001 - keep the shipped x86-64-v3 build
+        _NATIVE_NOTE = f"shipped -march=x86-64-v3 build (native build failed: {type(e).__name__})"
+    return _NATIVE_NOTE
+
+
 def lib():
     global _LIB
     if _LIB is None:
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
+        _load(path)
+    return _LIB
+
+
+def _load(path):
+    global _LIB
+    if _LIB is None:
         L = ctypes.CDLL(path)
         vp, sz, ci = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
         for suf in ("f64", "f32"):
@@ -56,6 +82,10 @@ def lib():
         L.oracle_imdct_brute.argtypes = [vp, vp, sz, vp]
         L.oracle_time_c2c.argtypes = [ci, vp, vp, sz, sz, ci, ci, ci]
         L.oracle_time_c2c.restype = ctypes.c_double
+        L.oracle_time_op.argtypes = [ci, ci, vp, vp, sz, sz, sz, sz, vp, ci, ci]
+        L.oracle_time_op.restype = ctypes.c_double
+        L.oracle_time_lens.argtypes = [ci, vp, vp, vp, sz, ci, ci]
+        L.oracle_time_lens.restype = ctypes.c_double
         _LIB = L
     return _LIB
 
@@ -246,3 +276,35 @@ def time_c2c(x, method="forward", threads=0, reps=3):
     n = x.shape[-1]
     return lib().oracle_time_c2c(int(x.dtype == np.complex128), _p(x), _p(out), n, x.size // n,
                                  _method(method), _threads(threads), reps)
+
+
+def time_op(kind, x, n, threads=0, reps=3, m=1, batch=None, hop=None, window=None):
+    """Seconds per pass of the CPU baseline for the other transform types
+    (plans built before the clock): kind "rfft" (x real [batch, n]), "dct2",
+    "dct3" (x real [batch, n]), "mdct" (x a 1-D signal, `batch` frames `hop`
+    apart, `window` of n entries), "c2c2d" (x complex [batch, m, n])."""
+    code = {"rfft": 1, "dct2": 2, "dct3": 3, "mdct": 4, "c2c2d": 5}[kind]
+    x = np.ascontiguousarray(x)
+    f64 = x.dtype in (np.float64, np.complex128)
+    rdt = np.float64 if f64 else np.float32
+    if kind == "rfft":
+        batch = x.size // n
+        out = np.empty((batch, n // 2 + 1), dtype=np.complex128 if f64 else np.complex64)
+    elif kind == "mdct":
+        out = np.empty((batch, n // 2), dtype=rdt)
+    else:
+        batch = x.size // (n * m)
+        out = np.empty_like(x)
+    w = None if window is None else np.ascontiguousarray(window, dtype=rdt)
+    return lib().oracle_time_op(code, int(f64), _p(x), _p(out), n, m, batch, hop or n,
+                                None if w is None else _p(w), _threads(threads), reps)
+
+
+def time_lens(x, lens, threads=0, reps=3):
+    """Seconds per pass of the CPU baseline over transforms of uneven lengths
+    packed back to back in x (CFft1D::forward of each, plans built first)."""
+    x = np.ascontiguousarray(x)
+    lv = np.ascontiguousarray(np.asarray(lens, dtype=np.uint64))
+    out = np.empty_like(x)
+    return lib().oracle_time_lens(int(x.dtype == np.complex128), _p(x), _p(out), _p(lv), len(lv),
+                                  _threads(threads), reps)

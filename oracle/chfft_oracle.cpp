@@ -1174,12 +1174,132 @@ static double time_c2c(const T* in, T* out, size_t n, size_t batch, int method, 
   return best;
 }
 
+// CPU baseline timer for the other transform types: `reps` passes of
+// run(plan, b) over b in [0, batch) on `threads` workers, each worker with its
+// own plan made by make() before the clock starts. Best seconds per pass.
+template <class P, class Make, class Run>
+static double time_batch(size_t batch, int threads, int reps, Make make, Run run) {
+  using clk = std::chrono::steady_clock;
+  if (threads < 1) threads = 1;
+  std::vector<P> plans;
+  plans.reserve(threads);
+  for (int t = 0; t < threads; t++) plans.push_back(make());
+  double best = 1e30;
+  for (int r = 0; r < reps; r++) {
+    auto t0 = clk::now();
+    std::vector<std::thread> ts;
+    size_t per = (batch + threads - 1) / threads;
+    for (int t = 0; t < threads; t++) {
+      size_t b0 = t * per, b1 = std::min(batch, b0 + per);
+      if (b0 >= b1) break;
+      ts.emplace_back([&, t, b0, b1] {
+        for (size_t b = b0; b < b1; b++) run(plans[t], b);
+      });
+    }
+    for (auto& th : ts) th.join();
+    auto t1 = clk::now();
+    best = std::min(best, std::chrono::duration<double>(t1 - t0).count());
+  }
+  return best;
+}
+
+template <class T>
+static double time_op(int kind, const T* in, T* out, size_t n, size_t m, size_t batch, size_t stride,
+                      const T* window, int threads, int reps) {
+  switch (kind) {
+    case 1:  // RFft1D::forward, n reals -> n/2 + 1 complex
+      return time_batch<RFft1D<T>>(batch, threads, reps, [&] { RFft1D<T> f; f.setup(n); return f; },
+                                   [&](RFft1D<T>& f, size_t b) {
+                                     f.convert(in + b * n, reinterpret_cast<Cx<T>*>(out) + b * (n / 2 + 1), T(1));
+                                   });
+    case 2:  // Dct1D type II forward
+      return time_batch<Dct2<T>>(batch, threads, reps, [&] { Dct2<T> d; d.setup(n, false); return d; },
+                                 [&](Dct2<T>& d, size_t b) { d.convert(in + b * n, out + b * n, T(1)); });
+    case 3:  // Dct1D type III forward
+      return time_batch<Dct3<T>>(batch, threads, reps, [&] { Dct3<T> d; d.setup(n, false); return d; },
+                                 [&](Dct3<T>& d, size_t b) { d.convert(in + b * n, out + b * n, T(1)); });
+    case 4:  // Mdct1D::forward of frames `stride` samples apart
+      if (!window) return -1.0;
+      return time_batch<Mdct<T>>(batch, threads, reps, [&] { Mdct<T> d; d.setup(n, window); return d; },
+                                 [&](Mdct<T>& d, size_t b) {
+                                   d.convert(in + b * stride, out + b * (n / 2), std::sqrt(T(n >> 1)));
+                                 });
+    case 5:  // CFft2D::forward of m x n matrices (the 2-D convert builds its plans per call)
+      return time_batch<int>(batch, threads, reps, [] { return 0; }, [&](int&, size_t b) {
+        cfft2d_convert(reinterpret_cast<const Cx<T>*>(in) + b * m * n, reinterpret_cast<Cx<T>*>(out) + b * m * n, m,
+                       n, false, T(1));
+      });
+  }
+  return -1.0;
+}
+
+// CPU baseline timer of a batch of uneven lengths (CFft1D::forward of each
+// transform, packed back to back): one plan per transform, built on the
+// workers before the clock starts; transforms are spread over the workers
+// longest first (greedy by n log n). Best seconds per pass.
+template <class T>
+static double time_lens(const T* in, T* out, const size_t* lens, size_t count, int threads, int reps) {
+  using clk = std::chrono::steady_clock;
+  if (threads < 1) threads = 1;
+  std::vector<size_t> off(count + 1, 0), order(count);
+  for (size_t i = 0; i < count; i++) {
+    off[i + 1] = off[i] + lens[i];
+    order[i] = i;
+  }
+  auto cost = [&](size_t i) { return (double)lens[i] * std::log2((double)lens[i] + 1.0); };
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
+  std::vector<std::vector<size_t>> part(threads);
+  std::vector<double> load(threads, 0.0);
+  for (size_t i : order) {
+    int t = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    part[t].push_back(i);
+    load[t] += cost(i);
+  }
+  std::vector<CFft1D<T>> plans(count);
+  parallel_for((size_t)threads, threads, [&](size_t t0, size_t t1) {
+    for (size_t t = t0; t < t1; t++)
+      for (size_t i : part[t]) plans[i].setup(lens[i]);
+  });
+  double best = 1e30;
+  for (int r = 0; r < reps; r++) {
+    auto a = clk::now();
+    parallel_for((size_t)threads, threads, [&](size_t t0, size_t t1) {
+      for (size_t t = t0; t < t1; t++)
+        for (size_t i : part[t])
+          plans[i].convert(reinterpret_cast<const Cx<T>*>(in) + off[i], reinterpret_cast<Cx<T>*>(out) + off[i],
+                           lens[i], false, T(1));
+    });
+    auto b = clk::now();
+    best = std::min(best, std::chrono::duration<double>(b - a).count());
+  }
+  return best;
+}
+
 extern "C" {
 double oracle_time_c2c(int f64, const void* in, void* out, size_t n, size_t batch, int method, int threads,
                        int reps) {
   if (n < 2) return -1.0;
   return f64 ? time_c2c((const double*)in, (double*)out, n, batch, method, threads, reps)
              : time_c2c((const float*)in, (float*)out, n, batch, method, threads, reps);
+}
+
+// kind: 1 RFft1D forward, 2 DCT-II, 3 DCT-III, 4 MDCT forward (frames
+// `stride` samples apart; window of n entries in the same precision),
+// 5 CFft2D forward (m x n).
+double oracle_time_op(int kind, int f64, const void* in, void* out, size_t n, size_t m, size_t batch, size_t stride,
+                      const void* window, int threads, int reps) {
+  if (n < 2) return -1.0;
+  return f64 ? time_op(kind, (const double*)in, (double*)out, n, m, batch, stride, (const double*)window, threads,
+                       reps)
+             : time_op(kind, (const float*)in, (float*)out, n, m, batch, stride, (const float*)window, threads, reps);
+}
+
+double oracle_time_lens(int f64, const void* in, void* out, const size_t* lens, size_t count, int threads,
+                        int reps) {
+  for (size_t i = 0; i < count; i++)
+    if (lens[i] == 0) return -1.0;
+  return f64 ? time_lens((const double*)in, (double*)out, lens, count, threads, reps)
+             : time_lens((const float*)in, (float*)out, lens, count, threads, reps);
 }
 
 }  // extern "C"
