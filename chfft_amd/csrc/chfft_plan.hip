@@ -64,6 +64,7 @@ struct SfftEntry {
 };
 const SfftEntry* sfft_find(bool f64, int L, int ld, int st, int mode, bool col);
 bool sfft_has_schedule(bool f64, int L, const int* rad, int nrad);
+int sfft_fused_bluestein_m(bool f64, long long N);
 hipError_t sfft_launch(const SfftEntry* e, const KParams& P, dim3 grid, size_t lds, hipStream_t st);
 hipError_t sfft_set_lds_limits();
 size_t sfft_lds_bytes(const SfftEntry* e);
@@ -544,17 +545,20 @@ bool fusable(size_t L, bool f64) {
   return 2 * next_pow2(L) <= cap;
 }
 
-int build_inner(size_t L, bool f64, Inner& in) {
+// M_special > 0: a fused Bluestein over that padded length, which only a
+// length-specialised kernel runs (its tables follow that kernel's
+// prime-factor schedule); else M = 2 * next_pow2(L) as cfft1d.rs:114.
+int build_inner(size_t L, bool f64, Inner& in, size_t M_special = 0) {
   in.L = L;
   if (smooth7(L)) {
     in.Lfft = L;
     in.bluestein = false;
     return build_seq(L, f64, in.seq, in.tw);
   }
-  size_t M = 2 * next_pow2(L);
+  size_t M = M_special ? M_special : 2 * next_pow2(L);
   in.Lfft = M;
   in.bluestein = true;
-  int rc = build_seq(M, f64, in.seq, in.tw);
+  int rc = build_seq(M, f64, in.seq, in.tw, M_special != 0);
   if (rc) return rc;
   if ((rc = upload(chirp_table(L), f64, in.chirp))) return rc;
   return upload(bluestein_spectrum(L, M), f64, in.kspec);
@@ -696,6 +700,15 @@ struct C2CEngine {
       mode = 1;
       return build_inner(len, f64, inner);
     }
+    // a non-7-smooth length whose padded Bluestein sequence a specialised
+    // kernel holds in one CU's LDS: one fused launch instead of three
+    // (N = 10007 over M = 20160 in FP32)
+    if (!smooth7(len) && !force_split && !force_persistent)
+      if (const int Ms = sfft_fused_bluestein_m(f64, (long long)len)) {
+        mode = 1;
+        big_fused = true;
+        return build_inner(len, f64, inner, (size_t)Ms);
+      }
     mode = smooth7(len) ? 2 : 3;
     const size_t total = mode == 2 ? len : bluestein_length(len, f64);
     M = total;
@@ -772,6 +785,9 @@ struct C2CEngine {
       l.what = std::string(inner.bluestein ? "fused bluestein N=" + std::to_string(n) + " M " : "fused ") +
                seq_str(inner.seq);
       try_specialise(l, f64);
+      // the generic fused kernel cannot hold such a length; the planner only
+      // chooses it where the specialised kernel applies (plain row tiles)
+      if (big_fused && l.kind != 3) l.kind = -1;
       out.push_back(l);
       return;
     }
@@ -946,6 +962,7 @@ struct C2CEngine {
   mutable int p_block = 0;
   mutable size_t p_lds = 0;
   bool force_split = false, force_persistent = false;
+  bool big_fused = false;  // mode 1 over a length only a specialised kernel runs
 
   void emit_persistent(std::vector<Launch>& out, size_t batch, bool inverse, int src, int dst, long long dist_in,
                        long long dist_out) const {
@@ -1240,6 +1257,12 @@ Launch transpose_launch(int src, int dst, size_t R, size_t C, size_t batch) {
 }
 
 int finish_plan(chfft_plan_s* p, chfft_plan* out) {
+  for (const std::vector<Launch>* L : {&p->fwd, &p->bwd})
+    for (const Launch& l : *L)
+      if (l.kind < 0) {  // a schedule no kernel runs (see C2CEngine::emit)
+        delete p;
+        return CHFFT_ERR_UNSUPPORTED;
+      }
   size_t need = 0;
   if (p->kind == K_C2C) need = p->rows.scratch_elems(p->batch);
   if (p->kind == K_C2C2D) {

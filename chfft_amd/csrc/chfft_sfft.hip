@@ -89,8 +89,14 @@ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 constexpr bool tw_in_lds(int entries, int esz) {
   return entries > 0 && entries * esz <= (esz == 8 ? CHFFT_TW_LDS_BYTES_F32 : CHFFT_TW_LDS_BYTES_F64);
 }
+// LDS padding: one slot per 2^pad_shift elements. A single transform longer
+// than 16384 points (a whole CU's LDS for one sequence) takes one slot per 64
+// so that its padded image still fits 160 KB; every other tile one per 16.
+constexpr int pad_shift(int L, int tpb) { return (tpb == 1 && L > 16384) ? 6 : 4; }
 constexpr int lds_stride(int L, int tpb) {
-  return (tpb > 1 && ((L + (L + 15) / 16) & 1) == 0) ? L + (L + 15) / 16 + 1 : L + (L + 15) / 16;
+  const int ps = pad_shift(L, tpb);
+  const int lp = L + ((L + (1 << ps) - 1) >> ps);
+  return (tpb > 1 && (lp & 1) == 0) ? lp + 1 : lp;
 }
 // LDS (complex elements) of the MDCT fold's shared tables: the window (4L
 // reals, as 2L pairs) and the L fold twiddles, read by every transform of a
@@ -148,6 +154,16 @@ struct LdsRows {
     return lxs<S>(base, j);
   }
 };
+// Rows padded one slot per 64 elements (pad_shift 6): element n at n + n / 64.
+struct LdsRows64 {
+  template <int S>
+  __device__ __forceinline__ static int at(int base, int j) {
+    if constexpr (S % 64 == 0)
+      return base + (base >> 6) + j * (S + S / 64);
+    else
+      return (base + j * S) + ((base + j * S) >> 6);
+  }
+};
 template <int TPB>
 struct LdsInterleaved {
   template <int S>
@@ -158,6 +174,11 @@ struct LdsInterleaved {
 
 template <class RS, int L, int TPT>
 struct Plan {
+  static constexpr int LEN = L;
+  // long single-transform tiles hold several butterflies per thread in a
+  // full register budget: their butterflies are computed one after the
+  // other (a scheduling fence between them) instead of interleaved
+  static constexpr bool SERIAL = L > 16384;
   static constexpr int NP = rad_count(RS{});
   static constexpr int R(int q) { return rad_at(RS{}, q); }
   static constexpr int P(int q) { return rad_span(RS{}, q); }
@@ -179,7 +200,12 @@ struct Plan {
 template <typename T, class PL, class RS, int q, int TPT>
 struct TwReg {
   static constexpr int R = PL::R(q), p = PL::P(q), B = PL::B(q);
-  static constexpr int NG = p <= 1 ? 0 : (TPT % p == 0 ? 1 : B);
+  static constexpr int NG0 = p <= 1 ? 0 : (TPT % p == 0 ? 1 : B);
+  // LAZY (long single-transform tiles): a pass whose butterflies need
+  // different twiddles reads them per butterfly inside pass_compute instead
+  // of holding the whole set in registers
+  static constexpr bool LAZY = PL::SERIAL && NG0 > 1;
+  static constexpr int NG = LAZY ? 0 : NG0;
   static constexpr int N = NG * (R - 1);
   // small sets are fetched one pass early, before the previous pass computes
   static constexpr bool AHEAD = N * (int)sizeof(cx<T>) <= CHFFT_TW_AHEAD_BYTES;
@@ -209,20 +235,29 @@ struct TwReg {
 };
 
 template <typename T, class PL, class RS, int q, int TPT>
-__device__ __forceinline__ void pass_compute(cx<T>* x, int u, const TwReg<T, PL, RS, q, TPT>& W) {
+__device__ __forceinline__ void pass_compute(cx<T>* x, int u, const TwReg<T, PL, RS, q, TPT>& W,
+                                             const cx<T>* __restrict__ tw) {
   using TR = TwReg<T, PL, RS, q, TPT>;
   constexpr int R = PL::R(q), NB = PL::NB(q), B = PL::B(q);
 #pragma unroll
   for (int b = 0; b < B; b++) {
     const int i = u + b * TPT;
     if (PL::FULL(q) || i < NB) {
-      if constexpr (TR::NG > 0) {
+      if constexpr (TR::LAZY) {
+        // full table [j-1][k] of this pass in global memory, k = i mod p
+        constexpr int off = rad_twoff(RS{}, q);
+        const unsigned k = (unsigned)i % (unsigned)TR::p;
+#pragma unroll
+        for (int j = 1; j < R; j++)
+          x[b * R + j] = x[b * R + j] * ld_off<cx<T>>(tw + off + (j - 1) * TR::p, k * (unsigned)sizeof(cx<T>));
+      } else if constexpr (TR::NG > 0) {
         constexpr int gs = TR::NG == 1 ? 0 : 1;
 #pragma unroll
         for (int j = 1; j < R; j++) x[b * R + j] = x[b * R + j] * W.w[gs * b * (R - 1) + j - 1];
       }
       dft<T, R>(x + b * R);
     }
+    if constexpr (PL::SERIAL && B > 1) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -239,6 +274,7 @@ __device__ __forceinline__ void pass_write(const cx<T>* x, int u, cx<T>* row) {
 #pragma unroll
       for (int j = 0; j < R; j++) row[LY::template at<p>(o, j)] = x[b * R + j];
     }
+    if constexpr (PL::SERIAL) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -253,6 +289,7 @@ __device__ __forceinline__ void pass_read(cx<T>* x, int u, const cx<T>* row) {
 #pragma unroll
       for (int j = 0; j < R; j++) x[b * R + j] = row[LY::template at<NB>(i, j)];
     }
+    if constexpr (PL::SERIAL) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -282,13 +319,13 @@ __device__ __forceinline__ void run_seq(cx<T>* x, int u, const cx<T>* __restrict
     using TN = TwReg<T, PL, RS, q + 1, TPT>;
     TN Wn;
     if constexpr (!LDT && TN::AHEAD) Wn.template load<false>(u, tw);
-    pass_compute<T, PL, RS, q, TPT>(x, u, W);
+    pass_compute<T, PL, RS, q, TPT>(x, u, W, tw);
     if constexpr (!LDT && !TN::AHEAD) Wn.template load<false>(u, tw);
     exchange<T, PL, q, q + 1, TPT, PP, LY>(x, u, row0, row1, cur);
     if constexpr (LDT) Wn.template load<true>(u, tw);
     run_seq<T, PL, RS, q + 1, TPT, PP, LY, LDT>(x, u, tw, row0, row1, cur, Wn);
   } else {
-    pass_compute<T, PL, RS, q, TPT>(x, u, W);
+    pass_compute<T, PL, RS, q, TPT>(x, u, W, tw);
   }
 }
 
@@ -804,7 +841,10 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
   const bool valid = g < P.G;
   // the epilogue reads the final image as padded rows
   constexpr bool IL = COL && !EPI;
-  using LY = std::conditional_t<IL, LdsInterleaved<TPB>, LdsRows>;
+  static_assert(!PL::SERIAL || !LDT, "per-butterfly twiddles read the full tables in global memory");
+  static_assert(pad_shift(L, TPB) == 4 || (!EPI && !IL && MTAB == 0),
+                "the epilogues and operator tables assume one padding slot per 16 elements");
+  using LY = std::conditional_t<IL, LdsInterleaved<TPB>, std::conditional_t<pad_shift(L, TPB) == 6, LdsRows64, LdsRows>>;
   cx<T>* row0 = IL ? lds + t : lds + t * LP;
   cx<T>* row1 = PP ? (IL ? lds + TPB * LP + t : lds + (TPB + t) * LP) : row0;
   const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.seq.tw);
@@ -849,9 +889,14 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
         for (int j = 0; j < RL; j++)
           x[b * RL + j] = cj(x[b * RL + j] * ld_off<cx<T>>(K, (unsigned)(i + j * PLAST) * ke));
       }
+      if constexpr (PL::SERIAL) __builtin_amdgcn_sched_barrier(0);
     }
-    exchange<T, PL, NP - 1, 0, TPT, PP, LY>(x, u, row0, row1, cur);
-    run_all<T, PL, RS, TPT, PP, LY, LDT>(x, u, twp, row0, row1, cur);
+    // long tiles: the second sequence recomputes its LDS and table
+    // addresses instead of keeping the first sequence's live in registers
+    int u2 = u;
+    if constexpr (PL::SERIAL) asm volatile("" : "+v"(u2));
+    exchange<T, PL, NP - 1, 0, TPT, PP, LY>(x, u2, row0, row1, cur);
+    run_all<T, PL, RS, TPT, PP, LY, LDT>(x, u2, twp, row0, row1, cur);
   }
   if constexpr (BLUE) {
 #pragma unroll
@@ -864,6 +909,7 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
           if (lower_half<RL>(j) && u + rel < P.L) x[b * RL + j] = cj(x[b * RL + j]) * ld_off<cx<T>>(chirp + rel, (unsigned)u * ESZ);
         }
       }
+      if constexpr (PL::SERIAL) __builtin_amdgcn_sched_barrier(0);
     }
   }
 
@@ -1149,6 +1195,12 @@ static const SfftEntry kSfft[] = {
     // 8192: fused Bluestein for N <= 4096 (N = 4093 and smaller primes), CFft1D
     SF_BOTH(8192, 512, 1, 1, false, LD_PLAIN, ST_PLAIN, 1, false, 4, 16, 16, 8, 4),
     SF_BOTH(8192, 512, 1, 1, false, LD_PLAIN, ST_PLAIN, 0, false, 4, 16, 16, 8, 4),
+    // 20160 = 2^6 3^2 5 7 (FP32): fused Bluestein for 8192 < N <= 10080 (N =
+    // 10007), the whole padded sequence (20475 slots) in one CU's LDS; 768
+    // threads (168 registers each; 960 threads spilled at 128), a multiple
+    // of the spans 16 and 192 so that those passes' twiddles repeat across a
+    // thread's butterflies
+    SF_ENTRY(0, float, 20160, 768, 1, false, LD_PLAIN, ST_PLAIN, 1, false, 4, 16, 12, 15, 7),
     // 43200 = 180 x 240 (rows of the column pass 128-byte aligned), two
     // passes each: 180 = 12 x 15, 240 = 16 x 15 (prime-factor radices)
     SF_BOTH(180, 16, 16, 8, true, LD_PLAIN, ST_TW, 0, false, 2, 12, 15),
@@ -1241,6 +1293,18 @@ const SfftEntry* sfft_find(bool f64, int L, int ld, int st, int mode, bool col) 
     if (e.f64 == (int)f64 && e.L == L && e.ld == ld && e.st == st && e.mode == mode && e.col == (int)col)
       return &e;
   return nullptr;
+}
+
+// Smallest length-specialised fused Bluestein (plain load and store, row
+// tiles) whose padded length M holds an N-point transform (M >= 2N - 1 and
+// N <= M / 2, the half its first and last passes skip); 0 if none.
+int sfft_fused_bluestein_m(bool f64, long long N) {
+  int best = 0;
+  for (const SfftEntry& e : kSfft)
+    if (e.f64 == (int)f64 && e.mode == 1 && e.ld == LD_PLAIN && e.st == ST_PLAIN && !e.col && e.L >= 2 * N - 1 &&
+        2 * N <= e.L && (best == 0 || e.L < best))
+      best = e.L;
+  return best;
 }
 
 // Whether some registered kernel of this length runs exactly this radix
