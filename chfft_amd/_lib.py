@@ -8,7 +8,8 @@ import subprocess
 _PKG = os.path.dirname(os.path.abspath(__file__))
 _ROOT = os.path.dirname(_PKG)
 LIB_PATH = os.path.join(_PKG, "libchfft_hip.so")
-SOURCES = ["csrc/chfft_kernels.hip", "csrc/chfft_sfft.hip", "csrc/chfft_tables.hip", "csrc/chfft_plan.hip"]
+SOURCES = ["csrc/chfft_kernels.hip", "csrc/chfft_sfft.hip", "csrc/chfft_multi.hip", "csrc/chfft_tables.hip",
+           "csrc/chfft_plan.hip"]
 COMPILE_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # no packed FP32 instructions (v_pk_*_f32) in any kernel

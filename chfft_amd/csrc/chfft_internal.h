@@ -87,6 +87,7 @@ enum StoreOp : int {
   ST_DCT3 = 5,
   ST_MDCT = 6,
   ST_IMDCT = 7,
+  ST_TW2 = 8,          // four-step twiddle from two-level tables (twsh), length-specialised tiles
 };
 
 struct KParams {
@@ -116,6 +117,7 @@ struct KParams {
   const void* kspec;
   const void* tab_a;
   const void* tab_b;
+  int spec;         // multi-length kernel: register-resident tile path of this descriptor (0: generic)
 };
 
 // One item of a multi-length batch (one transform). kind 0: fused (descs[desc]);

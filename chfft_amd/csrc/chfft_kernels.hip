@@ -21,148 +21,9 @@
 
 #include <algorithm>
 
-#include "chfft_device.h"
+#include "chfft_generic.h"
 
 namespace chfft {
-
-// ------------------------------------------------------------------ LDS passes
-// One Stockham pass (span p -> p*R) over all tpb sequences of the workgroup.
-// Butterfly i of a sequence reads x[i + j*L/R], multiplies input j by
-// W_{pR}^{j*k} (k = i mod p), and writes X_j to ((i-k)*R + k + j*p).
-// Inputs of all of a thread's butterflies are staged in registers between the
-// read barrier and the write, so one LDS buffer suffices.
-template <typename T, int R>
-__device__ __forceinline__ void lds_pass(cx<T>* lds, const TMap& m, int L, int Lp, const PassDesc& pd,
-                                         const cx<T>* __restrict__ tw) {
-  constexpr int BPT = (kEmax + R - 1) / R;
-  const int nbt = L / R;
-  const int p = pd.p;
-  cx<T> x[BPT][R];
-  int obase[BPT];
-  cx<T>* base = lds + m.tr * Lp;
-#pragma unroll
-  for (int q = 0; q < BPT; q++) {
-    const int i = m.ur + q * m.tpt;
-    obase[q] = -1;
-    if (i < nbt) {
-      const int k = (p == 1) ? 0 : i - qdiv(i, pd.pmagic) * p;
-#pragma unroll
-      for (int j = 0; j < R; j++) x[q][j] = base[lx(i + j * nbt)];
-      if (p > 1) {
-        // per-pass table laid out [j-1][k]: consecutive lanes read consecutive k
-        const cx<T>* twq = tw + pd.twoff + k;
-#pragma unroll
-        for (int j = 1; j < R; j++) x[q][j] = x[q][j] * twq[(j - 1) * p];
-      }
-      obase[q] = (i - k) * R + k;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < BPT; q++) {
-    if (obase[q] >= 0) {
-      dft<T, R>(x[q]);
-#pragma unroll
-      for (int j = 0; j < R; j++) base[lx(obase[q] + j * p)] = x[q][j];
-    }
-  }
-  __syncthreads();
-}
-
-template <typename T>
-__device__ void run_passes(cx<T>* lds, const TMap& m, const SeqDesc& s, int Lp) {
-  const cx<T>* tw = reinterpret_cast<const cx<T>*>(s.tw);
-  for (int q = 0; q < s.npass; q++) {
-    const PassDesc& pd = s.pass[q];
-    switch (pd.radix) {
-      case 2: lds_pass<T, 2>(lds, m, s.L, Lp, pd, tw); break;
-      case 3: lds_pass<T, 3>(lds, m, s.L, Lp, pd, tw); break;
-      case 4: lds_pass<T, 4>(lds, m, s.L, Lp, pd, tw); break;
-      case 5: lds_pass<T, 5>(lds, m, s.L, Lp, pd, tw); break;
-      case 7: lds_pass<T, 7>(lds, m, s.L, Lp, pd, tw); break;
-      case 8: lds_pass<T, 8>(lds, m, s.L, Lp, pd, tw); break;
-      case 16: lds_pass<T, 16>(lds, m, s.L, Lp, pd, tw); break;
-      default: break;
-    }
-  }
-}
-
-template <typename T, int LD>
-__device__ void load_stage(const KParams& P, const TMap& m, const void* src, cx<T>* lds, int g0, int nt) {
-  const int t = P.colmajor_ld ? m.tc : m.tr;
-  const int u = P.colmajor_ld ? m.uc : m.ur;
-  const cx<T>* chirp = reinterpret_cast<const cx<T>*>(P.chirp);
-  cx<T>* row = lds + t * P.Lp;
-  if (t < nt) {
-    const Loader<T, LD> ld = make_loader<T, LD>(P, src, g0 + t);
-    // batches of 8 elements per thread: all loads of a batch are issued
-    // before any LDS write, so they are in flight together
-    constexpr int NB = 8;
-    for (int n0 = u; n0 < P.Lfft; n0 += NB * m.tpt) {
-      cx<T> v[NB];
-#pragma unroll
-      for (int e = 0; e < NB; e++) {
-        const int n = n0 + e * m.tpt;
-        v[e] = mk(T(0), T(0));
-        if (n < P.L) v[e] = load_elem<T, LD>(P, ld, n);  // n >= L only under a fused Bluestein: zero padding
-      }
-#pragma unroll
-      for (int e = 0; e < NB; e++) {
-        const int n = n0 + e * m.tpt;
-        if (n < P.Lfft) {
-          cx<T> x = v[e];
-          if (n < P.L) {
-            if constexpr (LD != LD_FS_CHIRP) {
-              if (P.conj_in) x = cj(x);
-            }
-            if (P.has_chirp) x = x * chirp[n];
-          }
-          row[lx(n)] = x;
-        }
-      }
-    }
-  } else {
-    for (int n = u; n < P.Lfft; n += m.tpt) row[lx(n)] = mk(T(0), T(0));
-  }
-  __syncthreads();
-}
-
-// Bluestein spectrum product (chirpz.rs:115-117) followed by a conjugation so
-// that the inverse transform reuses the forward passes.
-template <typename T>
-__device__ void mid_stage(const KParams& P, const TMap& m, cx<T>* lds, int g0) {
-  const cx<T>* K = reinterpret_cast<const cx<T>*>(P.kspec);
-  const int c = (g0 + m.tr) % P.in.ncols;
-  const long long kb = (long long)c * P.kc;
-  cx<T>* row = lds + m.tr * P.Lp;
-  for (int n = m.ur; n < P.Lfft; n += m.tpt) row[lx(n)] = cj(row[lx(n)] * K[kb + (long long)n * P.ke]);
-  __syncthreads();
-}
-
-// Fused Bluestein de-chirp (chirpz.rs:127-145 with the conjugation undone).
-template <typename T>
-__device__ void dechirp_stage(const KParams& P, const TMap& m, cx<T>* lds) {
-  const cx<T>* chirp = reinterpret_cast<const cx<T>*>(P.chirp);
-  cx<T>* row = lds + m.tr * P.Lp;
-  for (int n = m.ur; n < P.L; n += m.tpt) row[lx(n)] = cj(row[lx(n)]) * chirp[n];
-  __syncthreads();
-}
-
-// One LDS-resident batch of tpb transforms: load, passes, [Bluestein middle],
-// store. The store stage is not followed by a barrier.
-template <typename T, int LD, int ST>
-__device__ __forceinline__ void block_transform(const KParams& P, const void* src, void* dst, cx<T>* lds, int g0,
-                                                int nt, double scale) {
-  const TMap m = make_map(P.tpb);
-  load_stage<T, LD>(P, m, src, lds, g0, nt);
-  run_passes<T>(lds, m, P.seq, P.Lp);
-  if (P.has_mid) {
-    mid_stage<T>(P, m, lds, g0);
-    run_passes<T>(lds, m, P.seq, P.Lp);
-    if (P.has_chirp) dechirp_stage<T>(P, m, lds);
-  }
-  store_stage<T, ST>(P, m, dst, lds, g0, nt, scale);
-}
 
 // FP64 workgroups are capped at 512 threads so that a thread may use up to 256
 // VGPRs (16 staged double-complex values without spilling).
@@ -213,135 +74,6 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? kMaxThreadsFsF64 : kMaxThread
       run_phase<T, LD_PLAIN, ST_FS_DECHIRP>(Q[2], slot, dst, lds, scale);
     }
   }
-}
-
-// Grid-wide barrier of the multi-length launch. The launch is cooperative
-// where the stream allows it (every workgroup resident, guaranteed by the
-// runtime) and otherwise sized to the occupancy, so a workgroup that arrives
-// waits only for the others to finish their share of the phase. The wait is
-// bounded: after `ticks` of the 100 MHz real-time counter (4 s by default)
-// the workgroup records the failure in sync[1], which stays set until the
-// host reads it (chfft_plan_check), and leaves the kernel, so no wave can
-// wait forever. `extra` raises the arrival target (0 in use; the host's test
-// mode sets it to make every wait time out). The agent-scope fences publish
-// each workgroup's scratch writes to every XCD before it arrives, and make
-// the others' writes visible after.
-__device__ __forceinline__ bool grid_barrier(unsigned* sync, unsigned target, unsigned long long ticks) {
-  __shared__ int ok;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    __hip_atomic_fetch_add(&sync[0], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    int good = 1;
-    const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-    while (__hip_atomic_load(&sync[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(8);
-      if (__builtin_amdgcn_s_memrealtime() - t0 > ticks) {
-        good = 0;
-        __hip_atomic_fetch_or(&sync[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-    }
-    ok = good;
-  }
-  __syncthreads();
-  __threadfence();
-  return ok != 0;
-}
-
-// Clears the barrier counter before each multi-length launch (a kernel rather
-// than a memset, so that the pair can be captured in a graph and replayed).
-// The failure flag sync[1] is left alone: the host clears it when it reads it.
-__global__ void multi_sync_reset(unsigned* sync) {
-  if (threadIdx.x == 0) sync[0] = 0u;
-}
-
-// Uneven multi-length batch in one persistent launch, in up to three phases
-// separated by grid-wide barriers: phase 0 runs the column passes of the
-// four-step items (chirp + columns for Bluestein), phase 1 their row passes
-// (spectrum product and inverse rows for Bluestein), phase 2 the inverse
-// columns + de-chirp of the Bluestein items; whole fused items (in LDS) fill
-// whichever phase the planner balanced them into. Within a phase, workgroup w
-// takes units w, w + grid, ... (the planner sorts each phase by decreasing
-// cost), so a long transform is spread over many workgroups instead of
-// bounding the launch. Every loop bound is a unit count and every barrier
-// wait is bounded, so every wave exits.
-template <typename T>
-__global__ void __launch_bounds__(sizeof(T) == 8 ? kMultiThreadsF64 : kMultiThreadsF32)
-    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 4)))
-    multi_kernel(const MultiItem* __restrict__ items, const MultiUnit* __restrict__ units, int end0, int end1,
-                 int end2, const KParams* __restrict__ descs, const void* in, void* out, int scale_kind,
-                 void* scratch, unsigned* sync, unsigned long long ticks, unsigned extra) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
-  const int ends[kMultiPhases] = {end0, end1, end2};
-  const int nph = end2 > end1 ? 3 : end1 > end0 ? 2 : 1;
-  int u0 = 0;
-  for (int ph = 0; ph < nph; ph++) {
-    for (int u = u0 + blockIdx.x; u < ends[ph]; u += gridDim.x) {
-      const MultiUnit U = units[u];
-      const MultiItem m = items[U.item];
-      const cx<T>* src = reinterpret_cast<const cx<T>*>(in) + m.off;
-      cx<T>* dst = reinterpret_cast<cx<T>*>(out) + m.off;
-      cx<T>* slot = reinterpret_cast<cx<T>*>(scratch) + m.soff;
-      const double n = (double)m.n;
-      const double scale = scale_kind == 0 ? 1.0 : scale_kind == 1 ? 1.0 / sqrt(n) : 1.0 / n;
-      if (m.kind == 0) {
-        block_transform<T, LD_PLAIN, ST_PLAIN>(descs[m.desc], src, dst, lds, 0, 1, scale);
-        __syncthreads();
-      } else if (m.kind == 3) {
-        // length 1: returned unchanged (cfft1d.rs:213-214)
-        if (threadIdx.x == 0) dst[0] = src[0];
-      } else {
-        const KParams& Q = descs[m.desc + U.ph];
-        for (int g = U.g0; g < U.g1; g += Q.tpb) {
-          const int nt = min(Q.tpb, U.g1 - g);
-          if (m.kind == 1) {
-            if (U.ph == 0)
-              block_transform<T, LD_PLAIN, ST_TW>(Q, src, slot, lds, g, nt, 1.0);
-            else
-              block_transform<T, LD_PLAIN, ST_PLAIN>(Q, slot, dst, lds, g, nt, scale);
-          } else {
-            if (U.ph == 0)
-              block_transform<T, LD_FS_CHIRP, ST_TW>(Q, src, slot, lds, g, nt, 1.0);
-            else if (U.ph == 1)
-              block_transform<T, LD_PLAIN, ST_TW>(Q, slot, slot, lds, g, nt, 1.0);
-            else
-              block_transform<T, LD_PLAIN, ST_FS_DECHIRP>(Q, slot, dst, lds, g, nt, scale);
-          }
-          __syncthreads();  // LDS reuse by the next tile
-        }
-      }
-    }
-    u0 = ends[ph];
-    if (ph + 1 < nph && !grid_barrier(sync, (unsigned)(ph + 1) * gridDim.x + extra, ticks)) return;
-  }
-}
-
-// coop: launch cooperatively (the runtime then refuses a grid whose
-// workgroups cannot all be resident at once); a stream under graph capture
-// takes the ordinary launch.
-hipError_t launch_multi(bool f64, const MultiItem* items, const MultiUnit* units, const int* ends,
-                        const KParams* descs, const void* in, void* out, int scale_kind, void* scratch,
-                        unsigned* sync, unsigned long long ticks, unsigned extra, bool coop, dim3 grid, size_t lds,
-                        hipStream_t st) {
-  hipLaunchKernelGGL(multi_sync_reset, dim3(1), dim3(64), 0, st, sync);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  const dim3 block(f64 ? kMultiThreadsF64 : kMultiThreadsF32);
-  const void* fn = f64 ? reinterpret_cast<const void*>(&multi_kernel<double>)
-                       : reinterpret_cast<const void*>(&multi_kernel<float>);
-  int e0 = ends[0], e1 = ends[1], e2 = ends[2];
-  void* args[] = {&items, &units, &e0, &e1, &e2, &descs, &in, &out, &scale_kind, &scratch, &sync, &ticks, &extra};
-  if (coop) return hipLaunchCooperativeKernel(fn, grid, block, args, (unsigned)lds, st);
-  return hipLaunchKernel(fn, grid, block, args, lds, st);
-}
-
-int multi_occupancy(bool f64, size_t lds) {
-  int n = 0;
-  hipError_t e = f64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, multi_kernel<double>, kMultiThreadsF64, lds)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, multi_kernel<float>, kMultiThreadsF32, lds);
-  return (e == hipSuccess && n > 0) ? n : 1;
 }
 
 // Plain copy used for length-1 transforms (cfft1d.rs:213-214 returns the input
@@ -547,12 +279,6 @@ hipError_t set_fused_lds_limits() {
                        reinterpret_cast<const void*>(&fourstep_kernel<float, 3>),
                        reinterpret_cast<const void*>(&fourstep_kernel<double, 2>),
                        reinterpret_cast<const void*>(&fourstep_kernel<double, 3>)};
-  const void* ms[2] = {reinterpret_cast<const void*>(&multi_kernel<float>),
-                       reinterpret_cast<const void*>(&multi_kernel<double>)};
-  for (const void* f : ms) {  // these keep a few bytes of static LDS (the barrier flag)
-    hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes - 64);
-    if (r != hipSuccess) e = r;
-  }
   for (const void* f : fs) {
     hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes);
     if (r != hipSuccess) e = r;

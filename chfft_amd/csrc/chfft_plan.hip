@@ -51,6 +51,10 @@ hipError_t launch_transpose(bool f64, const void* src, void* dst, int R, int C, 
 hipError_t launch_ola(bool f64, const void* frames, void* out, long long n, long long hop, long long count,
                       hipStream_t st);
 int multi_occupancy(bool f64, size_t lds);
+hipError_t multi_set_lds_limits();
+struct MultiSpec;
+const MultiSpec* multi_spec_find(int L, int ld, int st, int mode, bool col);
+void multi_spec_info(const MultiSpec* s, int* id, int* tpb, int* nrad, const int** rad, size_t* lds);
 hipError_t gen_roots(bool f64, const RootJob* jobs_dev, int njobs, void* arena, hipStream_t st);
 hipError_t gen_rot(const long long* ns_dev, int count, long long M, void* out, hipStream_t st);
 hipError_t put_spectra(bool f64, const void* src, int count, long long M, long long n1, const long long* offs_dev,
@@ -1168,7 +1172,9 @@ int ensure_runtime() {
   std::call_once(g_lds_once, [] {
     g_lds_err = set_fused_lds_limits();
     hipError_t e2 = sfft_set_lds_limits();
+    hipError_t e3 = multi_set_lds_limits();
     if (g_lds_err == hipSuccess) g_lds_err = e2;
+    if (g_lds_err == hipSuccess) g_lds_err = e3;
   });
   if (g_lds_err != hipSuccess) return CHFFT_ERR_HIP;
   void* base = nullptr;
@@ -1966,6 +1972,7 @@ struct SeqTab {  // a pass sequence whose table lives in the arena
   SeqDesc seq{};
   long long off = 0;  // arena offset of its table
   size_t count = 0;
+  bool compact = false;  // also the compact tables of the length-specialised tiles
 };
 
 struct LenInfo {
@@ -1975,6 +1982,10 @@ struct LenInfo {
   int seq = -1, s1 = -1, s2 = -1;  // indices into the SeqTab list
   long long chirp = -1, kspec = -1, tw_hi = -1, tw_lo = -1;
   int twsh = 0;
+  // register-resident tile paths of the phases (0: generic), their tile
+  // widths, SeqTab indices of their schedules and LDS bytes
+  int spec[3] = {0, 0, 0}, spec_tpb[3] = {0, 0, 0}, spec_seq[3] = {-1, -1, -1};
+  size_t spec_lds[3] = {0, 0, 0};
   int desc = 0;  // first descriptor of this length (per direction)
   int ndesc = 0;
 };
@@ -2036,15 +2047,21 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
 
   // ---- distinct lengths and their schedules
   std::vector<SeqTab> seqs;
-  std::map<size_t, int> seq_of_len;
+  std::map<std::vector<int>, int> seq_of_sched;  // key: {L, compact?, radices...}
   long long arena = 0;
-  auto seq_for = [&](size_t L) {
-    auto it = seq_of_len.find(L);
-    if (it != seq_of_len.end()) return it->second;
+  // pass sequence of length L with radix schedule `rad` (default: the
+  // runtime-radix schedule); `compact`: the length-specialised tiles' table
+  // layout, the compact tables (two-level where large) after the full ones
+  auto seq_for = [&](size_t L, std::vector<int> rad = {}, bool compact = false) {
+    if (rad.empty()) rad = radices_for(L);
+    std::vector<int> key = {(int)L, compact ? 1 : 0};
+    key.insert(key.end(), rad.begin(), rad.end());
+    auto it = seq_of_sched.find(key);
+    if (it != seq_of_sched.end()) return it->second;
     SeqTab t;
-    std::vector<int> rad = radices_for(L);
     t.seq.L = (int)L;
     t.seq.npass = (int)rad.size();
+    t.compact = compact;
     int pp = 1;
     size_t cnt = 0;
     for (size_t q = 0; q < rad.size(); q++) {
@@ -2057,11 +2074,18 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
       if (pp > 1) cnt += (size_t)(rad[q] - 1) * pp;
       pp *= rad[q];
     }
+    if (compact) {
+      pp = 1;
+      for (size_t q = 0; q < rad.size(); q++) {
+        cnt += (size_t)tw_compact_entries(rad[q], pp);
+        pp *= rad[q];
+      }
+    }
     t.count = cnt ? cnt : 1;
     t.off = arena;
     arena += (long long)t.count;
     seqs.push_back(t);
-    seq_of_len[L] = (int)seqs.size() - 1;
+    seq_of_sched[key] = (int)seqs.size() - 1;
     return (int)seqs.size() - 1;
   };
   std::map<size_t, LenInfo> info;
@@ -2115,6 +2139,27 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
         L.n2 = M / best;
         L.s1 = seq_for(L.n1);
         L.s2 = seq_for(L.n2);
+        if (!f64 && getenv("CHFFT_MULTI_GENERIC") == nullptr) {
+          // phases with a register-resident tile path: columns (+ chirp),
+          // Bluestein rows (spectrum product, inverse rows), inverse columns
+          const bool bl = !sm;
+          const int nph = bl ? 3 : 1;  // smooth items' transposing row phase stays generic
+          for (int ph = 0; ph < nph; ph++) {
+            const MultiSpec* sp =
+                ph == 0 ? multi_spec_find((int)L.n1, bl ? LD_FS_CHIRP : LD_PLAIN, ST_TW2, 0, true)
+                : ph == 1 ? multi_spec_find((int)L.n2, LD_PLAIN, ST_TW2, 2, false)
+                          : multi_spec_find((int)L.n1, LD_PLAIN, ST_FS_DECHIRP, 0, true);
+            if (!sp) continue;
+            int id, tpb, nrad;
+            const int* rad;
+            size_t slds;
+            multi_spec_info(sp, &id, &tpb, &nrad, &rad, &slds);
+            L.spec[ph] = id;
+            L.spec_tpb[ph] = tpb;
+            L.spec_lds[ph] = slds;
+            L.spec_seq[ph] = seq_for(ph == 1 ? L.n2 : L.n1, std::vector<int>(rad, rad + nrad), true);
+          }
+        }
         L.twsh = (pow2_ceil_log(M) + 1) / 2;
         const size_t nhi = (M + ((size_t)1 << L.twsh) - 1) >> L.twsh, nlo = (size_t)1 << L.twsh;
         L.tw_hi = arena;
@@ -2134,16 +2179,37 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
     }
     info[n] = L;
   }
-  for (const SeqTab& t : seqs) {
+  auto table_jobs = [&](const SeqTab& t) {
     int pp = 1;
+    long long c = t.off;
     for (int q = 0; q < t.seq.npass; q++) {
       const PassDesc& pd = t.seq.pass[q];
-      if (pp > 1)
+      if (pp > 1) {
         jobs.push_back({t.off + pd.twoff, (long long)(pd.radix - 1) * pp, (long long)pp * pd.radix, (long long)pp, 0,
                         0});
+        c = t.off + pd.twoff + (long long)(pd.radix - 1) * pp;
+      }
       pp *= pd.radix;
     }
-  }
+    if (!t.compact) return;
+    // compact tables: [j-1][k mod 16] = W_{pR}^(j k) and [j-1][k / 16] =
+    // W_{pR/16}^(j k) where two-level, else the full [j-1][k]
+    pp = 1;
+    for (int q = 0; q < t.seq.npass; q++) {
+      const int R = t.seq.pass[q].radix;
+      if (pp > 1 && tw_two_level(R, pp)) {
+        jobs.push_back({c, (long long)(R - 1) * 16, (long long)pp * R, 16, 0, 0});
+        c += (long long)(R - 1) * 16;
+        jobs.push_back({c, (long long)(R - 1) * (pp / 16), (long long)pp * R / 16, (long long)(pp / 16), 0, 0});
+        c += (long long)(R - 1) * (pp / 16);
+      } else if (pp > 1) {
+        jobs.push_back({c, (long long)(R - 1) * pp, (long long)pp * R, (long long)pp, 0, 0});
+        c += (long long)(R - 1) * pp;
+      }
+      pp *= R;
+    }
+  };
+  for (const SeqTab& t : seqs) table_jobs(t);
 
   // ---- tables on the device (FP64 math, rounded once)
   if ((rc = dev_alloc(p->arena, (size_t)std::max<long long>(arena, 1) * esz))) return rc;
@@ -2277,6 +2343,16 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
         }
         lds = std::max(lds, (size_t)ta * A.Lp * esz);
         lds = std::max(lds, (size_t)tb * B.Lp * esz);
+        KParams* Qs[3] = {&A, &B, &C};
+        for (int ph = 0; ph < 3; ph++)
+          if (L.spec[ph]) {
+            KParams& Q = *Qs[ph];
+            Q.spec = L.spec[ph];
+            Q.tpb = L.spec_tpb[ph];
+            Q.seq = seqs[L.spec_seq[ph]].seq;
+            Q.seq.tw = at(seqs[L.spec_seq[ph]].off);
+            lds = std::max(lds, L.spec_lds[ph]);
+          }
         D.push_back(A);
         D.push_back(B);
         if (L.kind == 2) D.push_back(C);
@@ -2375,16 +2451,19 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
   // every workgroup resident at once (the phases meet at grid-wide barriers)
   p->multi_grid = std::max(1, p->multi_occ * C2CEngine::num_cus());
   if ((rc = dev_alloc(p->scratch, (size_t)std::max<long long>(sacc, 1) * esz))) return rc;
-  size_t nf = 0, nf4 = 0, nb4 = 0;
+  size_t nf = 0, nf4 = 0, nb4 = 0, nspec = 0;
   for (auto& kv : info) {
     nf += kv.second.kind == 0;
     nf4 += kv.second.kind == 1;
     nb4 += kv.second.kind == 2;
+    for (int ph = 0; ph < 3; ph++) nspec += kv.second.spec[ph] != 0;
   }
   p->multi_desc = "multi-length batch: " + std::to_string(count) + " transforms, " + std::to_string(total) +
                   " points, " + std::to_string(info.size()) + " distinct lengths (" + std::to_string(nf) +
                   " fused in LDS, " + std::to_string(nf4) + " four-step, " + std::to_string(nb4) +
-                  " four-step Bluestein) | one persistent launch, " + std::to_string(nph) + " phase(s), " +
+                  " four-step Bluestein; " + std::to_string(nspec) +
+                  " four-step phases on register-resident tiles) | one persistent launch, " + std::to_string(nph) +
+                  " phase(s), " +
                   std::to_string(units.size()) + " work units: grid=" + std::to_string(p->multi_grid) +
                   " block=" + std::to_string(block) + " lds=" + std::to_string(p->multi_lds) +
                   " scratch=" + std::to_string((size_t)std::max<long long>(sacc, 1) * esz) + "\n";

@@ -51,9 +51,9 @@ def test_noalloc_matches_exec_two_calls(f64):
     for method in ("forward", "backward"):
         a, b = run_both(p1, x1, n1 * b1, cdt, method)
         assert torch.equal(a, b), method
-    # second call, smaller: a four-step Bluestein (10007 x 16), whose scratch
+    # second call, smaller: a four-step Bluestein (20011 x 16), whose scratch
     # overwrites the start of what the first call left
-    n2, b2 = 10007, 16
+    n2, b2 = 20011, 16
     p2 = Plan("c2c", f64, b2, n2)
     assert 0 < p2.scratch_bytes() < p1.scratch_bytes()
     x2 = cuda_normal(n2 * b2, cdt, 2)
