@@ -12,11 +12,11 @@ c2 the headline in both precisions, c3 Bluestein primes, c4 RFft/DCT/MDCT,
 c5 the uneven batch in one persistent launch and CFft2D 2048 x 2048, replayed
 through a HIP graph), one record per case with:
   * `roofline`: algorithmic bytes of the whole transform (one read of the
-    input, one write of the output) over the summed duration of the
-    schedule's kernel launches, each bracketed by HIP events on the stream it
-    is launched on; `traffic` = HBM bytes per step counted by rocprofv3
-    (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy) where a counter run has
-    been recorded, else null; `launches` lists each launch's average ms;
+    input, one write of the output) over the step time, HIP events on the
+    stream the kernels are launched on around K back-to-back steps;
+    `traffic` = bytes per step counted by rocprofv3 below the L2 (FETCH_SIZE
+    + WRITE_SIZE, scaled by a plain copy) where a counter run has been
+    recorded, else null; `launches_ms` brackets each launch on its own;
   * `cpu_baseline`: the oracle (a C++ restatement of chfft's algorithm,
     built with -march=native on this host where g++ is present) timed on a
     bounded sample of the same workload, plans built before the clock, on
@@ -168,15 +168,27 @@ def traffic_for(key):
     return None
 
 
-def roofline(alg_bytes, kms, key):
-    """Whole-transform roofline of one step: algorithmic bytes over the
-    summed launch durations (ms)."""
-    t = sum(kms)
-    ach = alg_bytes / (t * 1e6)
+def launch_shapes(plan, method="forward"):
+    """[grid, block, lds, load op, store op] of each launch of `method`'s
+    schedule (the key that ties rocprofv3's per-dispatch rows to this case)."""
+    import re
+    d = plan.describe()
+    part = d.split("inverse:")[0] if method.startswith("forward") else d.split("inverse:")[-1]
+    return [[int(g), int(b), int(l), int(a), int(c)]
+            for b, g, l, a, c in re.findall(r"block=(\d+) grid=(\d+) lds=(\d+) ops=(\d+)/(\d+)", part)]
+
+
+def roofline(alg_bytes, kms, key, ms_step):
+    """Whole-transform roofline of one step: algorithmic bytes over the step
+    time (ms_step: HIP events on the launch stream around K back-to-back
+    steps, which matches the summed kernel durations of a rocprofv3 trace;
+    `launches_ms` brackets each launch on its own and runs 10-30 % above the
+    trace, from the gaps the brackets add)."""
+    ach = alg_bytes / (ms_step * 1e6)
     tr = traffic_for(key)
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tr, "algorithmic_bytes": alg_bytes,
-            "kernel": f"whole schedule ({len(kms)} launch{'es' if len(kms) > 1 else ''} per step)",
+            "kernel": f"whole schedule ({len(kms)} launch{'es' if len(kms) > 1 else ''} per step)", "ms": ms_step,
             "launches_ms": [round(v, 5) for v in kms], "traffic_key": key}
 
 
@@ -235,9 +247,9 @@ def c2c_case(torch, n, batch, f64, args, seed, uniform=False, blocks=1, check=8,
     flop = 5.0 * n * math.log2(n)
     alg = (32.0 if f64 else 16.0) * n * batch
     rec = {"n": n, "batch": batch, "dtype": "f64" if f64 else "f32", "ms_per_step": round(ms, 5),
-           "gflops": round(flop * batch / (ms * 1e6), 2), "hbm_frac_step": round(alg / (ms * 1e6) / HBM_PEAK_GBS, 4),
-           "rel_l2": rel_l2(y, ref), "buffers_rotated": bufs.k,
-           "roofline": roofline(alg, kms, traffic_key or f"c2c_{n}x{batch}_{'f64' if f64 else 'f32'}")}
+           "gflops": round(flop * batch / (ms * 1e6), 2),
+           "rel_l2": rel_l2(y, ref), "buffers_rotated": bufs.k, "launch_shapes": launch_shapes(p),
+           "roofline": roofline(alg, kms, traffic_key or f"c2c_{n}x{batch}_{'f64' if f64 else 'f32'}", ms)}
     if cpu_note is not None:
         big = bufs.ins[0].view(tdt)[:batch * n].cpu().numpy().reshape(batch, n)
 
@@ -362,8 +374,8 @@ def cfg_c4(torch, args, cpu_note):
             flop = 2.5 * n * math.log2(n)
             rec = {"kind": kind, "n": n, "batch": batch, "dtype": "f64" if f64 else "f32",
                    "ms_per_step": round(ms, 5), "gflops": round(flop * batch / (ms * 1e6), 2),
-                   "rel_l2": rel_l2(got, ref), "buffers_rotated": b.k,
-                   "roofline": roofline(alg, kms, f"{kind}_{n}x{batch}_{'f64' if f64 else 'f32'}")}
+                   "rel_l2": rel_l2(got, ref), "buffers_rotated": b.k, "launch_shapes": launch_shapes(p),
+                   "roofline": roofline(alg, kms, f"{kind}_{n}x{batch}_{'f64' if f64 else 'f32'}", ms)}
             if cpu_note is not None and not f64:
                 full = b.ins[0].view(rdt)[:batch * n].cpu().numpy().reshape(batch, n)
 
@@ -401,8 +413,8 @@ def cfg_c4(torch, args, cpu_note):
         flop = 2.5 * n * math.log2(n)
         rec = {"kind": "mdct", "n": n, "hop": hop, "frames": frames, "dtype": "f64" if f64 else "f32",
                "ms_per_step": round(ms, 5), "gflops": round(flop * frames / (ms * 1e6), 2),
-               "rel_l2": rel_l2(got, ref), "buffers_rotated": kr,
-               "roofline": roofline(alg, kms, f"mdct_{n}x{frames}_{'f64' if f64 else 'f32'}")}
+               "rel_l2": rel_l2(got, ref), "buffers_rotated": kr, "launch_shapes": launch_shapes(p),
+               "roofline": roofline(alg, kms, f"mdct_{n}x{frames}_{'f64' if f64 else 'f32'}", ms)}
         if cpu_note is not None and not f64:
             w32 = oracle.window(n, "sine", np.float32)
 
@@ -487,10 +499,11 @@ def cfg_c5(torch, args, cpu_note):
           "ms_per_step": round(ms_multi, 4), "gflops": round(flop_multi / (ms_multi * 1e6), 2),
           "autotune_choice": choice, "rel_l2_per_bucket": bucket_err,
           "schedule": fft.plan.describe().strip().splitlines()[0],
-          "roofline": roofline(16.0 * fft.total, kms_multi, "multi_uneven10000_f32")}
+          "launch_shapes": [[int(fft.plan.describe().split("grid=")[1].split()[0]), 1024, 0, -1, -1]],
+          "roofline": roofline(16.0 * fft.total, kms_multi, "multi_uneven10000_f32", ms_multi)}
     r2 = {"part": "cfft2d", "m": 2048, "n": 2048, "dtype": "f32", "ms_per_step": round(ms_2d, 4),
-          "gflops": round(flop_2d / (ms_2d * 1e6), 2), "rel_l2": e2d,
-          "roofline": roofline(16.0 * 2048 * 2048, kms_2d, "c2c2d_2048x2048_f32")}
+          "gflops": round(flop_2d / (ms_2d * 1e6), 2), "rel_l2": e2d, "launch_shapes": launch_shapes(p2),
+          "roofline": roofline(16.0 * 2048 * 2048, kms_2d, "c2c2d_2048x2048_f32", ms_2d)}
     rg = {"part": "graph (multi + 2-D)", "ms_per_step": round(ms_graph, 4),
           "gflops": round((flop_multi + flop_2d) / (ms_graph * 1e6), 2)}
     if cpu_note is not None:

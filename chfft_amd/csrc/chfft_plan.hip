@@ -1949,8 +1949,10 @@ int chfft_plan_describe(chfft_plan plan, char* buf, size_t buf_len) {
     s += dir ? "inverse:\n" : "forward:\n";
     for (const Launch& l : L) {
       char line[512];
-      snprintf(line, sizeof line, "  %s | G=%d tpb=%d block=%u grid=%u lds=%zu\n", l.what.c_str(), l.P.G, l.P.tpb,
-               l.block.x, l.grid.x, l.lds);
+      // ops = load / store operator of the fused kernels (their template
+      // arguments, which tie a profiler's per-dispatch rows to the launch)
+      snprintf(line, sizeof line, "  %s | G=%d tpb=%d block=%u grid=%u lds=%zu ops=%d/%d\n", l.what.c_str(), l.P.G,
+               l.P.tpb, l.block.x, l.grid.x, l.lds, l.ld, l.st);
       s += line;
     }
   }
