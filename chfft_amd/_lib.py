@@ -70,6 +70,8 @@ def _signatures():
         "chfft_plan_check": ([vp, vp], ci),
         "chfft_exec_mdct_ola": ([vp, vp, vp, ci, vp], ci),
         "chfft_plan_autotune": ([vp, vp, vp, ci, vp], ci),
+        "chfft_plan_tune": ([vp, ctypes.c_char_p], ci),
+        "chfft_plan_tuning": ([vp, ctypes.c_char_p, ctypes.c_size_t], ci),
         "chfft_plan_destroy": ([vp], ci),
         "chfft_exec": ([vp, vp, vp, ci, vp], ci),
         "chfft_exec_noalloc": ([vp, vp, vp, ci, vp], ci),

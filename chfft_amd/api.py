@@ -51,7 +51,9 @@ def _real_dtype(dtype):
 class Plan:
     """Owning handle of one chfft_plan (a length or shape plus a batch count)."""
 
-    def __init__(self, kind, f64, batch, *args):
+    def __init__(self, kind, f64, batch, *args, tuned=True):
+        """`tuned`: apply the tuning table's spec for this shape, if it has
+        one (chfft_amd.tuning)."""
         self.kind = kind
         self.f64 = f64
         self.batch = batch
@@ -77,6 +79,10 @@ class Plan:
             raise ValueError(kind)
         _lib.check(st, f"plan {kind}{args}")
         self._h = h
+        self.tuned_spec = None
+        if tuned and kind in ("c2c", "c2c_multi"):
+            from . import tuning
+            self.tuned_spec = tuning.apply(self)
 
     @property
     def handle(self):
@@ -112,6 +118,20 @@ class Plan:
         if r < 0:
             _lib.check(-r, "autotune")
         return r
+
+    def tune(self, spec):
+        """Rebuild this plan under planner overrides (chfft_plan_tune), e.g.
+        "generic=1 order=2 tpb=4 tpt=64" or "n1=180 chunk=-1"; "default" is
+        the planner's own choice. Not while an exec of the plan is running."""
+        _lib.check(_lib.lib().chfft_plan_tune(self._h, spec.encode()), "tune")
+
+    def tuning(self):
+        """(spec, report): the overrides the plan runs with and the last
+        autotune's candidate times (chfft_plan_tuning)."""
+        buf = ctypes.create_string_buffer(16384)
+        _lib.check(_lib.lib().chfft_plan_tuning(self._h, buf, 16384), "tuning")
+        spec, _, report = buf.value.decode().partition("\n")
+        return spec, report
 
     def check(self, stream=0):
         """Wait for `stream` and raise ChfftError if a multi-length launch of

@@ -94,6 +94,21 @@ struct DevBuf {
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  // moves keep the device address (launch descriptors hold it)
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
@@ -326,6 +341,109 @@ std::vector<int> radices_for(size_t L, bool pfa = false) {
   return r;
 }
 
+// Alternative radix schedules of a 7-smooth L for the runtime-radix kernels
+// (the autotuner's radix-order knob): [0] the planner's schedule, then its
+// reverse, the power-of-two part as radix-8 or radix-4 passes, and the odd
+// radices first. Distinct schedules only, each within kMaxPass passes.
+std::vector<std::vector<int>> radix_variants(size_t L) {
+  std::vector<std::vector<int>> out;
+  auto add = [&](const std::vector<int>& r) {
+    if ((int)r.size() > kMaxPass) return;
+    for (const auto& o : out)
+      if (o == r) return;
+    out.push_back(r);
+  };
+  const std::vector<int> base = radices_for(L);
+  add(base);
+  add(std::vector<int>(base.rbegin(), base.rend()));
+  int a = 0;
+  for (size_t m = L; m > 1 && m % 2 == 0; m /= 2) a++;
+  std::vector<int> odd;
+  for (int r : base)
+    if (r & 1) odd.push_back(r);
+  for (int lb : {3, 2}) {  // radix 8, radix 4
+    std::vector<int> r(a / lb, 1 << lb);
+    if (a % lb) r.push_back(1 << (a % lb));
+    r.insert(r.end(), odd.begin(), odd.end());
+    add(r);
+  }
+  std::vector<int> odd_first = odd;
+  for (int r : base)
+    if (!(r & 1)) odd_first.push_back(r);
+  add(odd_first);
+  return out;
+}
+
+// Planner choices the autotuner varies; all zero is the planner's own rules.
+// Text form: "key=value" pairs separated by spaces or commas (tune_str).
+struct Tune {
+  int generic = 0;      // 1: runtime-radix kernels only (no length-specialised kernel)
+  int order = 0;        // radix_variants() index of each runtime-radix sequence (where it has one)
+  int tpb = 0, tpt = 0; // fused launches: transforms per workgroup, threads per transform
+  long long n1 = 0;     // four-step split N = n1 x n2
+  long long chunk = 0;  // transforms per chunk of the split four-step (-1: the whole batch)
+  int sched = 0;        // 1: one persistent four-step launch; 2: split four-step launches, also where
+                        // the planner would fuse a padded Bluestein sequence into one workgroup
+  int grid = 0;         // multi-length plans: workgroups of the persistent launch
+  int unit = 0;         // multi-length plans: points per work unit of a four-step phase
+};
+
+std::string tune_str(const Tune& t) {
+  std::string s;
+  auto put = [&](const char* k, long long v) {
+    if (v == 0) return;
+    if (!s.empty()) s += ' ';
+    s += std::string(k) + "=" + std::to_string(v);
+  };
+  put("generic", t.generic);
+  put("order", t.order);
+  put("tpb", t.tpb);
+  put("tpt", t.tpt);
+  put("n1", t.n1);
+  put("chunk", t.chunk);
+  put("sched", t.sched);
+  put("grid", t.grid);
+  put("unit", t.unit);
+  return s.empty() ? "default" : s;
+}
+
+int tune_parse(const char* s, Tune& t) {
+  t = Tune{};
+  if (!s) return CHFFT_ERR_INVALID_ARGUMENT;
+  std::string str(s);
+  for (char& c : str)
+    if (c == ',') c = ' ';
+  size_t i = 0;
+  while (i < str.size()) {
+    while (i < str.size() && str[i] == ' ') i++;
+    if (i >= str.size()) break;
+    size_t j = str.find(' ', i);
+    if (j == std::string::npos) j = str.size();
+    const std::string tok = str.substr(i, j - i);
+    i = j;
+    if (tok == "default") continue;
+    const size_t eq = tok.find('=');
+    if (eq == std::string::npos || eq + 1 >= tok.size()) return CHFFT_ERR_INVALID_ARGUMENT;
+    const std::string k = tok.substr(0, eq), v = tok.substr(eq + 1);
+    char* end = nullptr;
+    const long long x = strtoll(v.c_str(), &end, 10);
+    if (!end || *end != '\0') return CHFFT_ERR_INVALID_ARGUMENT;
+    const bool small = x >= -1 && x <= (1LL << 30);
+    if (!small) return CHFFT_ERR_INVALID_ARGUMENT;
+    if (k == "generic" && (x == 0 || x == 1)) t.generic = (int)x;
+    else if (k == "order" && x >= 0) t.order = (int)x;
+    else if (k == "tpb" && x >= 0) t.tpb = (int)x;
+    else if (k == "tpt" && x >= 0) t.tpt = (int)x;
+    else if (k == "n1" && x >= 0) t.n1 = x;
+    else if (k == "chunk") t.chunk = x;
+    else if (k == "sched" && x >= 0 && x <= 2) t.sched = (int)x;
+    else if (k == "grid" && x >= 0) t.grid = (int)x;
+    else if (k == "unit" && x >= 0) t.unit = (int)x;
+    else return CHFFT_ERR_INVALID_ARGUMENT;
+  }
+  return CHFFT_OK;
+}
+
 size_t esz_of(bool f64) { return f64 ? 16 : 8; }
 
 int max_threads(bool f64) { return f64 ? kMaxThreadsF64 : kMaxThreadsF32; }
@@ -383,6 +501,25 @@ Geometry choose_geometry(int Lfft, long long G, bool colmajor, bool f64, int max
   return g;
 }
 
+// Workgroup shape of tpb transforms of tpt threads each (the autotuner's
+// tile knobs); false where it does not fit the kernel: fewer threads per
+// transform than Lfft / kEmax, more threads than the kernel's limit, or more
+// LDS than one workgroup gets.
+bool geometry_with(int Lfft, int tpb, int tpt, bool f64, Geometry& g) {
+  if (tpb < 1 || tpt < 1) return false;
+  if (tpt < (Lfft + kEmax - 1) / kEmax || (long long)tpb * tpt > max_threads(f64)) return false;
+  int Lp = Lfft + (Lfft + 15) / 16;
+  if (tpb > 1 && (Lp & 1) == 0) Lp++;
+  const size_t lds = (size_t)tpb * Lp * esz_of(f64);
+  if (lds > kLdsBytes) return false;
+  g.tpb = tpb;
+  g.tpt = tpt;
+  g.Lp = Lp;
+  g.block = tpb * tpt;
+  g.lds = lds;
+  return true;
+}
+
 // ------------------------------------------------------------------ plan objects
 enum Kind { K_C2C = 0, K_R2C = 1, K_C2C2D = 2, K_DCT = 3, K_MDCT = 4, K_MULTI = 5 };
 // Buffers a launch reads and writes: the exec's input and output, the
@@ -424,8 +561,14 @@ struct Inner {
   DevBuf tw, chirp, kspec;
 };
 
-int build_seq(size_t L, bool f64, SeqDesc& seq, DevBuf& tw, bool pfa = false) {
+// `order` > 0 picks radix_variants(L)[order] for a runtime-radix sequence
+// (ignored where L has fewer variants or with `pfa`).
+int build_seq(size_t L, bool f64, SeqDesc& seq, DevBuf& tw, bool pfa = false, int order = 0) {
   std::vector<int> rad = radices_for(L, pfa);
+  if (!pfa && order > 0) {
+    const std::vector<std::vector<int>> v = radix_variants(L);
+    if ((size_t)order < v.size()) rad = v[order];
+  }
   if ((int)rad.size() > kMaxPass) return CHFFT_ERR_UNSUPPORTED;
   seq.L = (int)L;
   seq.npass = (int)rad.size();
@@ -552,17 +695,17 @@ bool fusable(size_t L, bool f64) {
 // M_special > 0: a fused Bluestein over that padded length, which only a
 // length-specialised kernel runs (its tables follow that kernel's
 // prime-factor schedule); else M = 2 * next_pow2(L) as cfft1d.rs:114.
-int build_inner(size_t L, bool f64, Inner& in, size_t M_special = 0) {
+int build_inner(size_t L, bool f64, Inner& in, size_t M_special = 0, int order = 0) {
   in.L = L;
   if (smooth7(L)) {
     in.Lfft = L;
     in.bluestein = false;
-    return build_seq(L, f64, in.seq, in.tw);
+    return build_seq(L, f64, in.seq, in.tw, false, order);
   }
   size_t M = M_special ? M_special : 2 * next_pow2(L);
   in.Lfft = M;
   in.bluestein = true;
-  int rc = build_seq(M, f64, in.seq, in.tw, M_special != 0);
+  int rc = build_seq(M, f64, in.seq, in.tw, M_special != 0, order);
   if (rc) return rc;
   if ((rc = upload(chirp_table(L), f64, in.chirp))) return rc;
   return upload(bluestein_spectrum(L, M), f64, in.kspec);
@@ -686,6 +829,7 @@ struct C2CEngine {
   SeqDesc x1{}, x2{};  // prime-factor schedules of n1, n2 (L = 0: none registered)
   DevBuf tw1, tw2, twx1, twx2, twN, chirp, kspec;
   size_t M = 0;
+  Tune tune;  // planner overrides (the autotuner's candidates)
 
   int init(size_t len, bool use_f64) {
     n = len;
@@ -696,18 +840,22 @@ struct C2CEngine {
       force_split = strcmp(e, "split") == 0;
       force_persistent = strcmp(e, "persistent") == 0;
     }
+    if (tune.sched) {
+      force_split = tune.sched == 2;
+      force_persistent = tune.sched == 1;
+    }
     if (len == 1) {
       mode = 0;
       return CHFFT_OK;
     }
     if (fusable(len, f64)) {
       mode = 1;
-      return build_inner(len, f64, inner);
+      return build_inner(len, f64, inner, 0, tune.order);
     }
     // a non-7-smooth length whose padded Bluestein sequence a specialised
     // kernel holds in one CU's LDS: one fused launch instead of three
     // (N = 10007 over M = 20160 in FP32)
-    if (!smooth7(len) && !force_split && !force_persistent)
+    if (!smooth7(len) && !force_split && !force_persistent && !tune.generic)
       if (const int Ms = sfft_fused_bluestein_m(f64, (long long)len)) {
         mode = 1;
         big_fused = true;
@@ -716,14 +864,19 @@ struct C2CEngine {
     mode = smooth7(len) ? 2 : 3;
     const size_t total = mode == 2 ? len : bluestein_length(len, f64);
     M = total;
-    const size_t best1 = fourstep_split(total, f64);
+    size_t best1 = fourstep_split(total, f64);
+    if (tune.n1) {  // any 7-smooth split whose factors one workgroup each holds
+      const size_t t1 = (size_t)tune.n1;
+      if (total % t1 || !smooth7(t1) || !smooth7(total / t1) || t1 > lds_cap(f64)) return CHFFT_ERR_INVALID_ARGUMENT;
+      best1 = t1;
+    }
     if (best1 == 0) return CHFFT_ERR_UNSUPPORTED;
     n1 = best1;
     n2 = total / best1;
-    if (n2 * 4 > lds_cap(f64)) return CHFFT_ERR_UNSUPPORTED;
+    if (n2 * 4 > lds_cap(f64)) return tune.n1 ? CHFFT_ERR_INVALID_ARGUMENT : CHFFT_ERR_UNSUPPORTED;
     int rc;
-    if ((rc = build_seq(n1, f64, s1, tw1))) return rc;
-    if ((rc = build_seq(n2, f64, s2, tw2))) return rc;
+    if ((rc = build_seq(n1, f64, s1, tw1, false, tune.order))) return rc;
+    if ((rc = build_seq(n2, f64, s2, tw2, false, tune.order))) return rc;
     if ((rc = build_pfa(n1, x1, twx1))) return rc;
     if ((rc = build_pfa(n2, x2, twx2))) return rc;
     if ((rc = upload(fourstep_table(n1, n2), f64, twN))) return rc;
@@ -785,10 +938,14 @@ struct C2CEngine {
       l.P.conj_in = ci;
       l.P.conj_out = ci;
       Geometry g = choose_geometry((int)inner.Lfft, (long long)batch, false, f64);
+      if (tune.tpb || tune.tpt) {  // checked when the plan is built (geometry_ok)
+        Geometry t;
+        if (geometry_with((int)inner.Lfft, tune.tpb ? tune.tpb : g.tpb, tune.tpt ? tune.tpt : g.tpt, f64, t)) g = t;
+      }
       finish(l, g, src, dst, 1);
       l.what = std::string(inner.bluestein ? "fused bluestein N=" + std::to_string(n) + " M " : "fused ") +
                seq_str(inner.seq);
-      try_specialise(l, f64);
+      if (!tune.generic) try_specialise(l, f64);
       // the generic fused kernel cannot hold such a length; the planner only
       // chooses it where the specialised kernel applies (plain row tiles)
       if (big_fused && l.kind != 3) l.kind = -1;
@@ -839,6 +996,8 @@ struct C2CEngine {
       if (v > 0) c = (size_t)v;
       if (v == -1) c = batch;
     }
+    if (tune.chunk > 0) c = (size_t)tune.chunk;
+    if (tune.chunk == -1) c = batch;
     return std::max<size_t>(1, std::min(c, batch));
   }
 
@@ -862,7 +1021,7 @@ struct C2CEngine {
       a.P.twc = 1;
       finish(a, choose_geometry((int)n1, a.P.G, true, f64), src, S_SCRATCH, 0);
       a.what = "four-step columns " + seq_str(s1) + " + twiddle";
-      try_specialise(a, f64, &x1);
+      if (!tune.generic) try_specialise(a, f64, &x1);
     out.push_back(a);
       // K2: rows (length n2) -> out[k1 + n1*k2]
       Launch b{};
@@ -876,7 +1035,7 @@ struct C2CEngine {
       b.P.conj_out = ci;
       finish(b, choose_geometry((int)n2, b.P.G, true, f64), S_SCRATCH, dst, 1);
       b.what = "four-step rows " + seq_str(s2) + " + transpose";
-      try_specialise(b, f64, &x2);
+      if (!tune.generic) try_specialise(b, f64, &x2);
     out.push_back(b);
       return;
     }
@@ -898,7 +1057,7 @@ struct C2CEngine {
     a.P.twc = 1;
     finish(a, choose_geometry((int)n1, a.P.G, true, f64), src, S_SCRATCH, 0);
     a.what = "bluestein N=" + std::to_string(n) + " chirp + columns " + seq_str(s1) + " + twiddle";
-    try_specialise(a, f64, &x1);
+    if (!tune.generic) try_specialise(a, f64, &x1);
     out.push_back(a);
     Launch b{};
     b.ld = LD_PLAIN;
@@ -917,7 +1076,7 @@ struct C2CEngine {
     b.P.twc = (int)n2;
     finish(b, choose_geometry((int)n2, b.P.G, false, f64), S_SCRATCH, S_SCRATCH, 0);
     b.what = "bluestein rows " + seq_str(s2) + " x spectrum, inverse rows + twiddle";
-    try_specialise(b, f64, &x2);
+    if (!tune.generic) try_specialise(b, f64, &x2);
     out.push_back(b);
     Launch c{};
     c.ld = LD_PLAIN;
@@ -932,7 +1091,7 @@ struct C2CEngine {
     c.P.conj_out = ci;
     finish(c, choose_geometry((int)n1, c.P.G, true, f64), S_SCRATCH, dst, 1);
     c.what = "bluestein inverse columns " + seq_str(s1) + " + de-chirp";
-    try_specialise(c, f64, &x1);
+    if (!tune.generic) try_specialise(c, f64, &x1);
     out.push_back(c);
   }
 
@@ -1015,7 +1174,7 @@ struct C2CEngine {
       C.chirp = chirp.p;
       C.conj_out = ci;
     }
-    if (const SfsEntry* e = sfs_find(f64, (int)n1, (int)n2, l.nph)) {
+    if (const SfsEntry* e = tune.generic ? nullptr : sfs_find(f64, (int)n1, (int)n2, l.nph)) {
       int threads, tpb[3], Lp[3];
       size_t lds;
       sfs_info(e, &threads, tpb, Lp, &lds);
@@ -1133,6 +1292,8 @@ struct chfft_plan_s {
   unsigned long long barrier_ticks = kBarrierTicks;
   unsigned barrier_extra = 0;
   std::string multi_desc;
+  Tune tune;                // the planner overrides the plan was built with
+  std::string tune_report;  // the last autotune's candidates and times
 };
 
 namespace chfft {
@@ -1301,6 +1462,48 @@ int finish_plan(chfft_plan_s* p, chfft_plan* out) {
   return CHFFT_OK;
 }
 
+// A CFft1D plan of `batch` transforms of `len` points under the planner
+// overrides `t` (all zero: the planner's own choices). Overrides a plan
+// cannot take are CHFFT_ERR_INVALID_ARGUMENT: tile knobs on a four-step
+// length or a shape the fused kernel cannot run, a radix-order index the
+// fused sequence does not have, a split that does not divide the length,
+// a workgroup count.
+int c2c_create(size_t len, size_t batch, bool f64, const Tune& t, chfft_plan* out) {
+  if (t.grid) return CHFFT_ERR_INVALID_ARGUMENT;
+  chfft_plan_s* p = new chfft_plan_s();
+  p->kind = K_C2C;
+  p->f64 = f64;
+  p->n = len;
+  p->batch = batch;
+  p->tune = t;
+  p->rows.tune = t;
+  int rc = p->rows.init(len, f64);
+  if (!rc) {
+    const C2CEngine& e = p->rows;
+    const bool fs = e.mode == 2 || e.mode == 3;
+    if ((t.n1 || t.chunk || t.sched) && !fs) rc = CHFFT_ERR_INVALID_ARGUMENT;
+    if ((t.tpb || t.tpt) && e.mode != 1) rc = CHFFT_ERR_INVALID_ARGUMENT;
+    if (t.order && e.mode == 0) rc = CHFFT_ERR_INVALID_ARGUMENT;
+    if (e.mode == 1) {
+      Geometry g = choose_geometry((int)e.inner.Lfft, (long long)batch, false, f64), gt;
+      if ((t.tpb || t.tpt) && !geometry_with((int)e.inner.Lfft, t.tpb ? t.tpb : g.tpb, t.tpt ? t.tpt : g.tpt, f64, gt))
+        rc = CHFFT_ERR_INVALID_ARGUMENT;
+      if (t.order && (e.big_fused || (size_t)t.order >= radix_variants(e.inner.Lfft).size()))
+        rc = CHFFT_ERR_INVALID_ARGUMENT;
+    }
+    if (fs && t.order &&
+        (size_t)t.order >= std::max(radix_variants(e.n1).size(), radix_variants(e.n2).size()))
+      rc = CHFFT_ERR_INVALID_ARGUMENT;
+  }
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  p->rows.emit(p->fwd, batch, false, S_IN, S_OUT, (long long)len, (long long)len);
+  p->rows.emit(p->bwd, batch, true, S_IN, S_OUT, (long long)len, (long long)len);
+  return finish_plan(p, out);
+}
+
 }  // namespace
 }  // namespace chfft
 
@@ -1328,18 +1531,7 @@ int chfft_plan_c2c_1d(chfft_plan* plan, size_t len, size_t batch, int precision)
   if (batch == 0 || batch * len > (size_t)1 << 31) return CHFFT_ERR_INVALID_ARGUMENT;
   int rc = ensure_runtime();
   if (rc) return rc;
-  chfft_plan_s* p = new chfft_plan_s();
-  p->kind = K_C2C;
-  p->f64 = precision == CHFFT_F64;
-  p->n = len;
-  p->batch = batch;
-  if ((rc = p->rows.init(len, p->f64))) {
-    delete p;
-    return rc;
-  }
-  p->rows.emit(p->fwd, batch, false, S_IN, S_OUT, (long long)len, (long long)len);
-  p->rows.emit(p->bwd, batch, true, S_IN, S_OUT, (long long)len, (long long)len);
-  return finish_plan(p, plan);
+  return c2c_create(len, batch, precision == CHFFT_F64, Tune{}, plan);
 }
 
 int chfft_plan_c2c_2d(chfft_plan* plan, size_t len_m, size_t len_n, size_t batch, int precision) {
@@ -2023,9 +2215,15 @@ int lp_for(size_t L, int tpb) {
 }  // namespace
 }  // namespace chfft
 
-extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t count, int precision) {
+// The multi-length plan under the planner overrides `t`: `generic` keeps
+// every four-step phase on the runtime-radix tiles, `grid` sets the
+// persistent launch's workgroup count (at most every workgroup that fits at
+// once), `unit` the points per work unit of a four-step phase.
+static int multi_create(const size_t* lens, size_t count, int precision, const Tune& t, chfft_plan* plan) {
   if (!plan) return CHFFT_ERR_INVALID_ARGUMENT;
   *plan = nullptr;
+  if (t.tpb || t.tpt || t.order || t.n1 || t.chunk || t.sched || (t.unit && t.unit < 1024))
+    return CHFFT_ERR_INVALID_ARGUMENT;
   if (check_prec(precision)) return CHFFT_ERR_INVALID_ARGUMENT;
   if (count > 0 && !lens) return CHFFT_ERR_INVALID_ARGUMENT;
   size_t total = 0;
@@ -2046,6 +2244,7 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
   p->lens.assign(lens, lens + count);
   p->total = total;
   p->batch = 1;
+  p->tune = t;
 
   // ---- distinct lengths and their schedules
   std::vector<SeqTab> seqs;
@@ -2141,7 +2340,7 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
         L.n2 = M / best;
         L.s1 = seq_for(L.n1);
         L.s2 = seq_for(L.n2);
-        if (!f64 && getenv("CHFFT_MULTI_GENERIC") == nullptr) {
+        if (!f64 && !t.generic && getenv("CHFFT_MULTI_GENERIC") == nullptr) {
           // phases with a register-resident tile path: columns (+ chirp),
           // Bluestein rows (spectrum product, inverse rows), inverse columns
           const bool bl = !sm;
@@ -2384,7 +2583,7 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
   // whole fused items; fused items go to the phase with the least work so
   // far; each phase sorted by decreasing cost (workgroup w takes units w,
   // w + grid, ...)
-  const double kChunk = 32768.0;
+  const double kChunk = t.unit ? (double)t.unit : 32768.0;
   std::vector<std::pair<double, MultiUnit>> ph_units[kMultiPhases];
   double ph_cost[kMultiPhases] = {0, 0, 0};
   int nph = 1;
@@ -2452,6 +2651,8 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
   p->multi_occ = multi_occupancy(f64, p->multi_lds);
   // every workgroup resident at once (the phases meet at grid-wide barriers)
   p->multi_grid = std::max(1, p->multi_occ * C2CEngine::num_cus());
+  if (t.grid > p->multi_grid) return CHFFT_ERR_INVALID_ARGUMENT;
+  if (t.grid) p->multi_grid = t.grid;
   if ((rc = dev_alloc(p->scratch, (size_t)std::max<long long>(sacc, 1) * esz))) return rc;
   size_t nf = 0, nf4 = 0, nb4 = 0, nspec = 0;
   for (auto& kv : info) {
@@ -2473,6 +2674,10 @@ extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t
   return CHFFT_OK;
 }
 
+extern "C" int chfft_plan_c2c_multi(chfft_plan* plan, const size_t* lens, size_t count, int precision) {
+  return multi_create(lens, count, precision, Tune{}, plan);
+}
+
 // Sets the number of workgroups of a multi-length plan's persistent launch
 // (the autotuner's knob); 0 restores the default, every workgroup that fits
 // at once. Larger grids are refused: the launch's phases meet at grid-wide
@@ -2482,6 +2687,7 @@ extern "C" int chfft_plan_set_grid(chfft_plan plan, int grid) {
   const int cap = std::max(1, plan->multi_occ * C2CEngine::num_cus());
   if (grid > cap) return CHFFT_ERR_INVALID_ARGUMENT;
   plan->multi_grid = grid ? grid : cap;
+  plan->tune.grid = grid == cap ? 0 : grid;
   return CHFFT_OK;
 }
 
@@ -2489,93 +2695,216 @@ extern "C" int chfft_plan_set_grid(chfft_plan plan, int grid) {
 namespace chfft {
 namespace {
 
-// Rebuild the launches of a c2c plan with a four-step schedule override.
-int rebuild_c2c(chfft_plan_s* p, bool split, bool persistent) {
-  p->rows.force_split = split;
-  p->rows.force_persistent = persistent;
-  p->fwd.clear();
-  p->bwd.clear();
-  p->qbufs.clear();
-  p->rows.emit(p->fwd, p->batch, false, S_IN, S_OUT, (long long)p->n, (long long)p->n);
-  p->rows.emit(p->bwd, p->batch, true, S_IN, S_OUT, (long long)p->n, (long long)p->n);
-  size_t need = p->rows.scratch_elems(p->batch);
-  need = std::max(need, p->rows.persistent_scratch_elems(p->fwd));
-  need = std::max(need, p->rows.persistent_scratch_elems(p->bwd));
-  int rc = dev_alloc(p->scratch, need * esz_of(p->f64));
+// Rebuilds plan `p` under the overrides `t` (CFft1D and multi-length plans;
+// the other kinds take only the planner's own choices). The plan keeps its
+// handle; its buffers and launches are replaced once the device is idle.
+int apply_tune(chfft_plan_s* p, const Tune& t) {
+  chfft_plan fresh = nullptr;
+  int rc;
+  if (p->kind == K_C2C)
+    rc = c2c_create(p->n, p->batch, p->f64, t, &fresh);
+  else if (p->kind == K_MULTI)
+    rc = multi_create(p->lens.data(), p->lens.size(), p->f64 ? CHFFT_F64 : CHFFT_F32, t, &fresh);
+  else
+    return tune_str(t) == "default" ? CHFFT_OK : CHFFT_ERR_UNSUPPORTED;
   if (rc) return rc;
-  for (std::vector<Launch>* L : {&p->fwd, &p->bwd})
-    for (Launch& l : *L)
-      if (l.kind == 2 || l.kind == 4) {
-        p->qbufs.emplace_back(new DevBuf());
-        DevBuf& q = *p->qbufs.back();
-        if ((rc = dev_alloc(q, sizeof(KParams) * l.nph))) return rc;
-        if (hipMemcpy(q.p, l.Q, sizeof(KParams) * l.nph, hipMemcpyHostToDevice) != hipSuccess) return CHFFT_ERR_HIP;
-        l.qdev = q.p;
-      }
+  if (hipDeviceSynchronize() != hipSuccess) {
+    delete fresh;
+    return CHFFT_ERR_HIP;
+  }
+  std::string report = std::move(p->tune_report);
+  *p = std::move(*fresh);
+  p->tune_report = std::move(report);
+  delete fresh;
   return CHFFT_OK;
 }
 
+// Aligned four-step splits of `total` other than n1 (n2 a multiple of 16,
+// both factors held by one workgroup), nearest to the square root first.
+std::vector<size_t> alt_splits(size_t total, size_t n1, bool f64, size_t maxn) {
+  std::vector<size_t> v;
+  for (size_t d = 2; d <= lds_cap(f64) && d < total; d++) {
+    if (total % d || d == n1 || !smooth7(d) || !smooth7(total / d)) continue;
+    if ((total / d) % 16 || (total / d) * 4 > lds_cap(f64)) continue;
+    v.push_back(d);
+  }
+  const double mid = 0.5 * std::log((double)total);
+  std::sort(v.begin(), v.end(), [&](size_t a, size_t b) {
+    return std::fabs(std::log((double)a) - mid) < std::fabs(std::log((double)b) - mid);
+  });
+  if (v.size() > maxn) v.resize(maxn);
+  return v;
+}
+
+// The autotuner's candidates for plan `p`, the planner's own choice first.
+// Fused lengths: every radix schedule of radix_variants() and workgroup
+// shapes around the planner's (half / double the transforms per workgroup,
+// the fewest threads per transform, half / double the threads) on the
+// runtime-radix kernel. Four-step lengths: the persistent schedule, the
+// runtime-radix kernels, chunk sizes, other splits and radix orders.
+// Multi-length plans: workgroup counts, work-unit sizes, runtime-radix tiles.
+std::vector<Tune> tune_candidates(const chfft_plan_s* p) {
+  std::vector<Tune> c(1);
+  if (p->kind == K_MULTI) {
+    const int def = std::max(1, p->multi_occ * C2CEngine::num_cus());
+    for (int g : {def / 2, def})
+      for (int u : {16384, 0, 65536}) {
+        Tune t;
+        t.grid = g == def ? 0 : std::max(1, g);
+        t.unit = u;
+        if (t.grid || t.unit) c.push_back(t);
+      }
+    if (!p->f64) {
+      Tune t;
+      t.generic = 1;
+      c.push_back(t);
+    }
+    return c;
+  }
+  if (p->kind != K_C2C) return c;
+  const C2CEngine& e = p->rows;
+  if (e.mode == 1 && e.big_fused) {  // the four-step Bluestein schedules instead
+    for (int k = 0; k < 3; k++) {
+      Tune t;
+      t.sched = k == 0 ? 2 : k == 1 ? 1 : 0;
+      t.generic = k == 2;
+      c.push_back(t);
+    }
+  } else if (e.mode == 1) {
+    const int Lf = (int)e.inner.Lfft;
+    const size_t nv = radix_variants((size_t)Lf).size();
+    for (size_t o = 0; o < nv; o++) {
+      Tune t;
+      t.generic = 1;
+      t.order = (int)o;
+      c.push_back(t);
+    }
+    const Geometry g = choose_geometry(Lf, (long long)p->batch, false, p->f64);
+    const int tmin = (Lf + kEmax - 1) / kEmax;
+    const int shapes[5][2] = {{g.tpb / 2, g.tpt}, {g.tpb * 2, g.tpt}, {g.tpb, tmin}, {g.tpb, g.tpt / 2}, {g.tpb, g.tpt * 2}};
+    for (const auto& sh : shapes) {
+      Geometry gt;
+      if (sh[0] == g.tpb && sh[1] == g.tpt) continue;
+      if (!geometry_with(Lf, sh[0], sh[1], p->f64, gt)) continue;
+      bool dup = false;
+      for (const Tune& o : c) dup |= o.tpb == sh[0] && o.tpt == sh[1];
+      if (dup) continue;
+      Tune t;
+      t.generic = 1;
+      t.tpb = sh[0];
+      t.tpt = sh[1];
+      c.push_back(t);
+    }
+  } else if (e.mode == 2 || e.mode == 3) {
+    Tune t;
+    t.sched = 1;
+    c.push_back(t);
+    t = Tune{};
+    t.generic = 1;
+    c.push_back(t);
+    const size_t cb = e.chunk_of(p->batch);
+    if (cb < p->batch) {
+      t = Tune{};
+      t.chunk = -1;
+      c.push_back(t);
+    }
+    if (cb > 1) {
+      t = Tune{};
+      t.chunk = (long long)(cb / 2);
+      c.push_back(t);
+    }
+    for (size_t d : alt_splits(e.mode == 2 ? e.n : e.M, e.n1, p->f64, 3)) {
+      t = Tune{};
+      t.n1 = (long long)d;
+      c.push_back(t);
+    }
+    const size_t nv = std::min(radix_variants(e.n1).size(), radix_variants(e.n2).size());
+    for (size_t o = 1; o < std::min<size_t>(nv, 3); o++) {
+      t = Tune{};
+      t.generic = 1;
+      t.order = (int)o;
+      c.push_back(t);
+    }
+  }
+  return c;
+}
+
+// Average time of one forward exec over `reps` back-to-back execs (after
+// one untimed exec), best of two such runs.
 double time_exec(chfft_plan_s* p, void* in, void* out, int reps, hipStream_t st) {
   hipEvent_t a, b;
-  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1;
-  double ms = -1;
-  if (chfft_exec(p, in, out, CHFFT_FORWARD, st) == CHFFT_OK && hipEventRecord(a, st) == hipSuccess) {
-    bool ok = true;
+  if (hipEventCreate(&a) != hipSuccess) return -1;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return -1;
+  }
+  double best = -1;
+  bool ok = chfft_exec(p, in, out, CHFFT_FORWARD, st) == CHFFT_OK;
+  for (int run = 0; run < 2 && ok; run++) {
+    ok = hipEventRecord(a, st) == hipSuccess;
     for (int r = 0; r < reps && ok; r++) ok = chfft_exec(p, in, out, CHFFT_FORWARD, st) == CHFFT_OK;
     float f = 0;
-    if (ok && hipEventRecord(b, st) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
-        hipEventElapsedTime(&f, a, b) == hipSuccess)
-      ms = f / reps;
+    ok = ok && hipEventRecord(b, st) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+         hipEventElapsedTime(&f, a, b) == hipSuccess;
+    if (ok && (best < 0 || f / reps < best)) best = f / reps;
   }
+  if (ok && chfft_plan_check(p, st) != CHFFT_OK) best = -1;  // a multi-length launch that did not complete
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  return ms;
+  return ok ? best : -1;
 }
 
 }  // namespace
 }  // namespace chfft
 
+extern "C" int chfft_plan_tune(chfft_plan plan, const char* spec) {
+  if (!plan || !spec) return CHFFT_ERR_INVALID_ARGUMENT;
+  Tune t;
+  if (tune_parse(spec, t)) return CHFFT_ERR_INVALID_ARGUMENT;
+  return apply_tune(plan, t);
+}
+
+extern "C" int chfft_plan_tuning(chfft_plan plan, char* buf, size_t buf_len) {
+  if (!plan || !buf || buf_len == 0) return CHFFT_ERR_INVALID_ARGUMENT;
+  const std::string s = tune_str(plan->tune) + "\n" + plan->tune_report;
+  const size_t n = std::min(s.size(), buf_len - 1);
+  memcpy(buf, s.data(), n);
+  buf[n] = '\0';
+  return CHFFT_OK;
+}
+
 extern "C" int chfft_plan_autotune(chfft_plan plan, void* in, void* out, int reps, void* stream) {
   if (!plan || !in || !out || reps <= 0) return -CHFFT_ERR_INVALID_ARGUMENT;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  std::string report = "autotune:";
+  const std::vector<Tune> cands = tune_candidates(plan);
+  std::string report = "autotune (" + std::to_string(reps) + " execs per candidate, ms per exec):";
   int best = -1;
   double best_ms = 1e30;
-  if (plan->kind == K_MULTI) {
-    const int def = std::max(1, plan->multi_occ * C2CEngine::num_cus());
-    const int cand[3] = {std::max(1, def / 4), std::max(1, def / 2), def};
-    for (int i = 0; i < 3; i++) {
-      const int g = cand[i];
-      if (chfft_plan_set_grid(plan, g) != CHFFT_OK) continue;
-      const double ms = time_exec(plan, in, out, reps, st);
-      report += " grid=" + std::to_string(g) + ":" + std::to_string(ms) + "ms";
-      if (ms > 0 && ms < best_ms) {
-        best_ms = ms;
-        best = i;
-      }
+  for (size_t i = 0; i < cands.size(); i++) {
+    chfft_plan fresh = nullptr;
+    const int rc = plan->kind == K_MULTI
+                       ? multi_create(plan->lens.data(), plan->lens.size(), plan->f64 ? CHFFT_F64 : CHFFT_F32,
+                                      cands[i], &fresh)
+                       : plan->kind == K_C2C ? c2c_create(plan->n, plan->batch, plan->f64, cands[i], &fresh)
+                                             : CHFFT_ERR_UNSUPPORTED;
+    if (rc) {
+      if (i == 0 && plan->kind != K_C2C && plan->kind != K_MULTI) return 0;  // a single schedule
+      continue;
     }
-    if (best < 0) return -CHFFT_ERR_HIP;
-    chfft_plan_set_grid(plan, cand[best]);
-    plan->multi_desc += report + " -> grid " + std::to_string(plan->multi_grid) + "\n";
-    return best;
-  }
-  if (plan->kind == K_C2C && (plan->rows.mode == 2 || plan->rows.mode == 3)) {
-    const bool opts[2][2] = {{true, false}, {false, true}};
-    const char* names[2] = {"split", "persistent"};
-    for (int i = 0; i < 2; i++) {
-      if (rebuild_c2c(plan, opts[i][0], opts[i][1]) != CHFFT_OK) continue;
-      const double ms = time_exec(plan, in, out, reps, st);
-      report += std::string(" ") + names[i] + ":" + std::to_string(ms) + "ms";
-      if (ms > 0 && ms < best_ms) {
-        best_ms = ms;
-        best = i;
-      }
+    const double ms = time_exec(fresh, in, out, reps, st);
+    (void)hipStreamSynchronize(st);
+    delete fresh;
+    char line[160];
+    snprintf(line, sizeof line, "\n  [%zu] %s: %.4f", i, tune_str(cands[i]).c_str(), ms);
+    report += line;
+    if (ms > 0 && ms < best_ms) {
+      best_ms = ms;
+      best = (int)i;
     }
-    if (best < 0) return -CHFFT_ERR_HIP;
-    int rc = rebuild_c2c(plan, opts[best][0], opts[best][1]);
-    if (rc) return -rc;
-    if (!plan->fwd.empty()) plan->fwd.back().what += " [" + report + "]";
-    return best;
   }
-  return 0;  // a single schedule: nothing to choose
+  if (best < 0) return -CHFFT_ERR_HIP;
+  const int rc = apply_tune(plan, cands[best]);
+  if (rc) return -rc;
+  plan->tune_report = report + "\n  -> [" + std::to_string(best) + "] " + tune_str(cands[best]);
+  return best;
 }

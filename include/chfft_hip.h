@@ -142,13 +142,48 @@ int chfft_plan_set_grid(chfft_plan plan, int grid);
    plan kinds: CHFFT_OK after the wait. */
 int chfft_plan_check(chfft_plan plan, void* stream);
 
-/* Autotuner (no counterpart in the reference): times the schedules this plan
-   can run — for a multi-length plan the persistent launch's workgroup count,
-   for a four-step length the split launches against the persistent
-   one-transform-per-workgroup launch — on the given device buffers (their
-   contents are overwritten), keeps the fastest and returns its index
-   (>= 0) or minus a status code. `reps` timed executions per candidate. */
+/* Autotuner (no counterpart in the reference; it takes the role of the
+   reference's fixed factor order, prime_factorization.rs:66-113): times
+   the schedules this plan can run on the given device buffers (their
+   contents are overwritten), rebuilds the plan with the fastest and returns
+   its index (>= 0; 0 is the planner's own choice) or minus a status code.
+   `reps` timed executions per candidate, best of two runs. Candidates, as
+   chfft_plan_tune specs: a fused length tries every radix order of the
+   runtime-radix kernel and workgroup shapes around the planner's (half /
+   double the transforms per workgroup, the fewest threads per transform,
+   half / double the threads per transform) against the length-specialised
+   kernel; a four-step length the persistent schedule, the runtime-radix
+   kernels, chunk sizes, other splits N = n1 x n2 and radix orders (a
+   Bluestein length fused into one workgroup: the four-step schedules); a
+   multi-length plan workgroup counts, work-unit sizes and the runtime-radix
+   tiles. CFft1D and multi-length plans; other kinds return 0. */
 int chfft_plan_autotune(chfft_plan plan, void* in, void* out, int reps, void* stream);
+
+/* Planner overrides (no counterpart in the reference): rebuilds a CFft1D or
+   multi-length plan under `spec`, "key=value" pairs separated by spaces or
+   commas ("default" or "" = the planner's own choice):
+     generic=1   runtime-radix kernels only (no length-specialised kernel)
+     order=k     radix schedule k of each runtime-radix sequence (0 = the
+                 planner's; fused lengths: an order the length has)
+     tpb=, tpt=  fused length: transforms per workgroup, threads per
+                 transform (threads per transform >= length / 16)
+     n1=         four-step split N = n1 x n2 (7-smooth factors one
+                 workgroup holds)
+     chunk=      four-step: transforms per chunk (-1 = the whole batch)
+     sched=1     four-step: one persistent launch
+     sched=2     four-step: split launches, also for a Bluestein length the
+                 planner would fuse into one workgroup
+     grid=, unit= multi-length: workgroups of the persistent launch,
+                 points per work unit of a four-step phase (>= 1024)
+   CHFFT_ERR_INVALID_ARGUMENT for a spec the plan cannot take (the plan is
+   left as it was); CHFFT_ERR_UNSUPPORTED for overrides on other plan kinds.
+   The device must be idle of this plan's execs (the call waits for the
+   device before replacing the plan's buffers). */
+int chfft_plan_tune(chfft_plan plan, const char* spec);
+
+/* The spec the plan was built with, a newline, and the last autotune's
+   report (every candidate's time), NUL-terminated, truncated to buf_len. */
+int chfft_plan_tuning(chfft_plan plan, char* buf, size_t buf_len);
 
 int chfft_plan_destroy(chfft_plan plan);
 
