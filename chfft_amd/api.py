@@ -103,8 +103,8 @@ class Plan:
         return _lib.lib().chfft_plan_scratch_bytes(self._h)
 
     def describe(self):
-        buf = ctypes.create_string_buffer(8192)
-        _lib.lib().chfft_plan_describe(self._h, buf, 8192)
+        buf = ctypes.create_string_buffer(65536)
+        _lib.lib().chfft_plan_describe(self._h, buf, 65536)
         return buf.value.decode()
 
     def autotune(self, src, dst, reps=3, stream=0):

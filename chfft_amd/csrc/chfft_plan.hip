@@ -2148,6 +2148,12 @@ int chfft_plan_describe(chfft_plan plan, char* buf, size_t buf_len) {
       s += line;
     }
   }
+  // the planner overrides the plan runs with, and the last autotune's report
+  const std::string spec = tune_str(plan->tune);
+  if (spec != "default" || !plan->tune_report.empty()) {
+    s += "tuning: " + spec + "\n";
+    if (!plan->tune_report.empty()) s += plan->tune_report + "\n";
+  }
   if (buf && buf_len) {
     size_t k = std::min(buf_len - 1, s.size());
     memcpy(buf, s.data(), k);

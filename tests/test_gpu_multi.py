@@ -120,7 +120,9 @@ def test_fourstep_autotune():
     xh = (r.standard_normal((batch, n)) + 1j * r.standard_normal((batch, n))).astype(np.complex64)
     x = torch.from_numpy(xh).cuda()
     y = torch.empty_like(x)
-    assert p.autotune(x, y, reps=2) in (0, 1)
+    best = p.autotune(x, y, reps=2)
+    spec, report = p.tuning()
+    assert 0 <= best and f"[{best}] {spec}:" in report, (best, spec, report)
     p.exec(x, y, "forward")
     torch.cuda.synchronize()
     e = rel_l2(y.cpu().numpy(), oracle.c2c(xh.astype(np.complex128), "forward"))
