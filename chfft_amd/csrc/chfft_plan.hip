@@ -69,6 +69,7 @@ struct SfftEntry {
 const SfftEntry* sfft_find(bool f64, int L, int ld, int st, int mode, bool col);
 bool sfft_has_schedule(bool f64, int L, const int* rad, int nrad);
 int sfft_fused_bluestein_m(bool f64, long long N);
+int sfft_schedule_of(bool f64, int L, int ld, int st, int mode, bool col, int* rad);
 hipError_t sfft_launch(const SfftEntry* e, const KParams& P, dim3 grid, size_t lds, hipStream_t st);
 hipError_t sfft_set_lds_limits();
 size_t sfft_lds_bytes(const SfftEntry* e);
@@ -563,9 +564,10 @@ struct Inner {
 
 // `order` > 0 picks radix_variants(L)[order] for a runtime-radix sequence
 // (ignored where L has fewer variants or with `pfa`).
-int build_seq(size_t L, bool f64, SeqDesc& seq, DevBuf& tw, bool pfa = false, int order = 0) {
-  std::vector<int> rad = radices_for(L, pfa);
-  if (!pfa && order > 0) {
+int build_seq(size_t L, bool f64, SeqDesc& seq, DevBuf& tw, bool pfa = false, int order = 0,
+              const std::vector<int>* given = nullptr) {
+  std::vector<int> rad = given ? *given : radices_for(L, pfa);
+  if (!given && !pfa && order > 0) {
     const std::vector<std::vector<int>> v = radix_variants(L);
     if ((size_t)order < v.size()) rad = v[order];
   }
@@ -705,7 +707,11 @@ int build_inner(size_t L, bool f64, Inner& in, size_t M_special = 0, int order =
   size_t M = M_special ? M_special : 2 * next_pow2(L);
   in.Lfft = M;
   in.bluestein = true;
-  int rc = build_seq(M, f64, in.seq, in.tw, M_special != 0, order);
+  // a padded length only a specialised kernel runs: that kernel's schedule
+  int rs[8];
+  const int nrs = M_special ? sfft_schedule_of(f64, (int)M, LD_PLAIN, ST_PLAIN, 1, false, rs) : 0;
+  const std::vector<int> given(rs, rs + nrs);
+  int rc = build_seq(M, f64, in.seq, in.tw, M_special != 0, order, nrs ? &given : nullptr);
   if (rc) return rc;
   if ((rc = upload(chirp_table(L), f64, in.chirp))) return rc;
   return upload(bluestein_spectrum(L, M), f64, in.kspec);
