@@ -658,14 +658,18 @@ size_t fourstep_split(size_t total, bool f64) {
 }
 
 // Padded length of a four-step Bluestein. The reference pads to
-// M = 2 * next_pow2(N) (cfft1d.rs:114); every M >= 2N - 1 holds the same
-// linear convolution of the chirped input with the chirp, so the results
-// agree to rounding. Taken here: the 5-smooth M in [2N - 1, 2 * next_pow2(N)]
+// M = 2 * next_pow2(N) (cfft1d.rs:114); every M >= 2N - 2 holds the same
+// linear convolution of the chirped input a (indices 0..N-1) with the chirp
+// filter h (indices -(N-1)..N-1), so the results agree to rounding: the
+// cyclic convolution over M reads h at (k - j) mod M for k, j < N, and the
+// only two lags that share a slot when M = 2N - 2 are N-1 and -(N-1), where
+// h is symmetric (h[m] = h[-m] = W^(-m^2/2)) and the slot holds the right
+// value for both. Taken here: the 5-smooth M in [2N - 2, 2 * next_pow2(N)]
 // with an aligned split (n2 % 16 == 0, n2 fusable) that needs the least work,
 // M * (passes of n1 + passes of n2), a factor counting its prime-factor
 // schedule where a specialised kernel runs it -- 20480 = 128 x 160 for
-// N = 10007 and, for N = 65537, 138240 = 360 x 384 (360 = 8 x 15 x 3, FP32)
-// or 147456 = 384 x 384 (FP64), instead of 32768 and 262144.
+// N = 10007 and, for N = 65537, 131072 = 256 x 512 = 2N - 2, instead of
+// 32768 and 262144.
 size_t bluestein_length(size_t N, bool f64) {
   const size_t hi = 2 * next_pow2(N);
   size_t best = hi;
@@ -675,7 +679,7 @@ size_t bluestein_length(size_t N, bool f64) {
     const size_t plain = radices_for(L).size();
     return sfft_has_schedule(f64, (int)L, rp.data(), (int)rp.size()) ? std::min(plain, rp.size()) : plain;
   };
-  for (size_t m = 2 * N - 1; m <= hi; m++) {
+  for (size_t m = 2 * N - 2; m <= hi; m++) {
     if (!smooth5(m)) continue;
     const size_t d = fourstep_split(m, f64);
     if (d == 0 || (m / d) % 16 != 0 || (m / d) * 4 > lds_cap(f64)) continue;

@@ -138,9 +138,8 @@ static const SfftEntry kSfft[] = {
     SF_BOTH(240, 16, 16, 8, false, LD_PLAIN, ST_PLAIN, 0, true, 2, 16, 15),
     // four-step Bluestein: chirp + columns, rows x spectrum + inverse rows,
     // inverse columns + de-chirp; M = 20480 = 128 x 160 (N = 10007),
-    // M = 32768 = 128 x 256, M = 138240 = 360 x 384 (N = 65537 in FP32;
-    // columns as radix 8 x 15 x 3), M = 147456 = 384 x 384 (N = 65537 in
-    // FP64)
+    // M = 32768 = 128 x 256, M = 138240 = 360 x 384 (columns as radix
+    // 8 x 15 x 3), M = 147456 = 384 x 384
     SF_BOTH(128, 16, 16, 16, true, LD_FS_CHIRP, ST_TW, 0, false, 2, 16, 8),
     SF_BOTH(160, 16, 8, 4, false, LD_PLAIN, ST_TW, 2, false, 2, 16, 10),
     SF_BOTH(256, 16, 4, 4, false, LD_PLAIN, ST_TW, 2, false, 2, 16, 16),
@@ -148,6 +147,11 @@ static const SfftEntry kSfft[] = {
     SF_BOTH(384, 32, 16, 8, true, LD_FS_CHIRP, ST_TW, 0, false, 3, 16, 8, 3),
     SF_BOTH(384, 32, 4, 4, false, LD_PLAIN, ST_TW, 2, false, 3, 16, 8, 3),
     SF_BOTH(384, 32, 16, 8, true, LD_PLAIN, ST_FS_DECHIRP, 0, false, 3, 16, 8, 3),
+    // M = 131072 = 256 x 512 = 2N - 2 (N = 65537): columns of 256 as
+    // radix 16 x 16, rows of 512 as 16 x 8 x 4
+    SF_BOTH(256, 16, 16, 8, true, LD_FS_CHIRP, ST_TW, 0, false, 2, 16, 16),
+    SF_BOTH(512, 32, 4, 4, false, LD_PLAIN, ST_TW, 2, false, 3, 16, 8, 4),
+    SF_BOTH(256, 16, 16, 8, true, LD_PLAIN, ST_FS_DECHIRP, 0, false, 2, 16, 16),
     // (FP32 only: in FP64 M = 138240 measured 3 % slower than 147456)
     SF_ENTRY(0, float, 360, 32, 16, true, LD_FS_CHIRP, ST_TW, 0, false, 3, 8, 15, 3),
     SF_ENTRY(0, float, 360, 32, 16, true, LD_PLAIN, ST_FS_DECHIRP, 0, false, 3, 8, 15, 3),

@@ -85,6 +85,26 @@ def test_c2c_config_sizes(n, batch, dtype):
     check(z, xr, f64, n)
 
 
+@pytest.mark.parametrize("n,cols,rows", [(65537, 256, 512), (16385, 128, 256), (40961, 256, 320)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_bluestein_padding_2n_minus_2(n, cols, rows, dtype):
+    # four-step Bluestein over M = 2N - 2 (2^17, 2^15, 2^14 x 5): the one
+    # slot that the lags N-1 and -(N-1) share holds the symmetric chirp's
+    # value for both, so the cyclic convolution equals the linear one
+    from chfft_amd import Plan
+    f64 = dtype == np.float64
+    cdt = np.complex128 if f64 else np.complex64
+    d = Plan("c2c", f64, 2, n).describe().split("inverse:")[0]
+    assert f"columns L={cols} " in d and f"rows L={rows} " in d and cols * rows == 2 * n - 2, d
+    r = rng(n + 7)
+    x = (r.standard_normal((2, n)) + 1j * r.standard_normal((2, n))).astype(cdt)
+    fft = CFft1D.with_len(n, dtype)
+    xr = x.astype(np.complex128)
+    for m in ("forward", "backward", "forwardn", "backwardu"):
+        check(getattr(fft, m)(x), np.fft.fft(xr) * {"forward": 1, "forwardn": 1 / n}[m] if m.startswith("f") else
+              np.fft.ifft(xr) * {"backward": 1, "backwardu": np.sqrt(n)}[m], f64, n)
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_rfft_sweep(dtype):
     # rfft1d.rs:390-402: even N = 2..198, all scalings, round trips
