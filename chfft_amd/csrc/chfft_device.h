@@ -16,6 +16,29 @@ struct alignas(2 * sizeof(T)) cx {
   T re, im;
 };
 
+// Store of one output value to global memory; NT: non-temporal (the L2
+// streams the line out instead of keeping it), for the exec's final output
+// in the kernels compiled for it.
+template <bool NT, typename T>
+__device__ __forceinline__ void gput(T* p, T v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void gput(cx<T>* p, cx<T> v) {
+  if constexpr (NT) {
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    V2 w;
+    w.x = v.re;
+    w.y = v.im;
+    __builtin_nontemporal_store(w, reinterpret_cast<V2*>(p));
+  } else {
+    *p = v;
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ cx<T> mk(T r, T i) {
   cx<T> c;
@@ -390,7 +413,7 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, const Loader<T, LD>
 // operator's own loop, written to the transform's output at element obase.
 // Shared by the fused kernels (Z reads the LDS image) and the stand-alone
 // post-pass kernel of the long-length schedules (Z reads global memory).
-template <typename T, int ST, class ZF>
+template <typename T, int ST, bool NT = false, class ZF>
 __device__ __forceinline__ void post_ops(const KParams& P, ZF Z, void* dst, long long obase, int i0, int step, T s) {
   if constexpr (ST == ST_R2C) {
     // rfft1d.rs:134-148: N/2+1 bins per row. Bins i and h - i (0 < i < q)
@@ -402,15 +425,15 @@ __device__ __forceinline__ void post_ops(const KParams& P, ZF Z, void* dst, long
     for (int i = i0; i <= q; i += step) {
       if (i == 0) {
         const cx<T> z = Z(0);
-        out[0] = scl(mk(z.re + z.im, T(0)), s);
-        out[h] = scl(mk(z.re - z.im, T(0)), s);
+        gput<NT>(&out[0], scl(mk(z.re + z.im, T(0)), s));
+        gput<NT>(&out[h], scl(mk(z.re - z.im, T(0)), s));
       } else if (i < q) {
         const cx<T> zi = Z(i), zh = Z(h - i);
         const cx<T> x = co[i] * (zi - cj(zh));
-        out[i] = scl(zi - x, s);
-        out[h - i] = scl(zh + cj(x), s);
+        gput<NT>(&out[i], scl(zi - x, s));
+        gput<NT>(&out[h - i], scl(zh + cj(x), s));
       } else if ((N & 3) == 0) {
-        out[q] = scl(cj(Z(q)), s);
+        gput<NT>(&out[q], scl(cj(Z(q)), s));
       }
     }
   } else if constexpr (ST == ST_DCT2) {
@@ -422,22 +445,22 @@ __device__ __forceinline__ void post_ops(const KParams& P, ZF Z, void* dst, long
     for (int i = i0; i <= q2; i += step) {
       if (i == 0) {
         const cx<T> z = Z(0);
-        ret[0] = (z.re + z.im) * s;
-        ret[h] = (z.re - z.im) * om[h].re * s;
+        gput<NT>(&ret[0], (z.re + z.im) * s);
+        gput<NT>(&ret[h], (z.re - z.im) * om[h].re * s);
       } else if (i < q2) {
         const cx<T> zi = Z(i);
         const cx<T> wc = cj(Z(h - i));
         const cx<T> x = co[i] * (zi - wc);
         const cx<T> j = cj(zi - x) * om[i];
-        ret[i] = j.re * s;
-        ret[N - i] = j.im * s;
+        gput<NT>(&ret[i], j.re * s);
+        gput<NT>(&ret[N - i], j.im * s);
         const cx<T> k = (wc + x) * om[h - i];
-        ret[h - i] = k.re * s;
-        ret[h + i] = k.im * s;
+        gput<NT>(&ret[h - i], k.re * s);
+        gput<NT>(&ret[h + i], k.im * s);
       } else if ((N & 3) == 0) {
         const cx<T> x = Z(q2) * om[q2];
-        ret[q2] = x.re * s;
-        ret[N - q2] = x.im * s;
+        gput<NT>(&ret[q2], x.re * s);
+        gput<NT>(&ret[N - q2], x.im * s);
       }
     }
   } else if constexpr (ST == ST_DCT3) {
@@ -449,13 +472,13 @@ __device__ __forceinline__ void post_ops(const KParams& P, ZF Z, void* dst, long
       const cx<T> a = Z(i);
       if (i < q2) {
         const cx<T> b = Z(h - 1 - i);
-        ret[4 * i] = a.re * s;
-        ret[4 * i + 1] = b.im * s;
-        ret[4 * i + 2] = a.im * s;
-        ret[4 * i + 3] = b.re * s;
+        gput<NT>(&ret[4 * i], a.re * s);
+        gput<NT>(&ret[4 * i + 1], b.im * s);
+        gput<NT>(&ret[4 * i + 2], a.im * s);
+        gput<NT>(&ret[4 * i + 3], b.re * s);
       } else {
-        ret[4 * i] = a.re * s;
-        ret[4 * i + 1] = a.im * s;
+        gput<NT>(&ret[4 * i], a.re * s);
+        gput<NT>(&ret[4 * i + 1], a.im * s);
       }
     }
     (void)N;
@@ -466,8 +489,8 @@ __device__ __forceinline__ void post_ops(const KParams& P, ZF Z, void* dst, long
     T* ret = reinterpret_cast<T*>(dst) + obase;
     for (int i = i0; i < q; i += step) {
       const cx<T> v = Z(i) * scl(tw[i], s);
-      ret[2 * i] = -v.re;
-      ret[h - 1 - 2 * i] = v.im;
+      gput<NT>(&ret[2 * i], -v.re);
+      gput<NT>(&ret[h - 1 - 2 * i], v.im);
     }
   } else {  // ST_IMDCT, mdct1d.rs:268-288
     const int N = (int)P.N, q = N >> 2, h = N >> 1, q3 = h + q, q5 = h + q3, lo = q - (q >> 1);
@@ -477,14 +500,14 @@ __device__ __forceinline__ void post_ops(const KParams& P, ZF Z, void* dst, long
     for (int i = i0; i < q; i += step) {
       const cx<T> v = Z(i) * scl(tw[i], s);
       const int mm = 2 * i;
-      ret[q3 - 1 - mm] = v.re * w[q3 - 1 - mm];
-      ret[q + mm] = -v.im * w[q + mm];
+      gput<NT>(&ret[q3 - 1 - mm], v.re * w[q3 - 1 - mm]);
+      gput<NT>(&ret[q + mm], -v.im * w[q + mm]);
       if (i < lo) {
-        ret[q3 + mm] = v.re * w[q3 + mm];
-        ret[q - 1 - mm] = v.im * w[q - 1 - mm];
+        gput<NT>(&ret[q3 + mm], v.re * w[q3 + mm]);
+        gput<NT>(&ret[q - 1 - mm], v.im * w[q - 1 - mm]);
       } else {
-        ret[mm - q] = -v.re * w[mm - q];
-        ret[q5 - 1 - mm] = -v.im * w[q5 - 1 - mm];
+        gput<NT>(&ret[mm - q], -v.re * w[mm - q]);
+        gput<NT>(&ret[q5 - 1 - mm], -v.im * w[q5 - 1 - mm]);
       }
     }
   }
@@ -493,7 +516,7 @@ __device__ __forceinline__ void post_ops(const KParams& P, ZF Z, void* dst, long
 
 // Store stage from the final LDS image (natural order, padded index), one
 // loop per store operator.
-template <typename T, int ST>
+template <typename T, int ST, bool NT = false>
 __device__ void store_stage(const KParams& P, const TMap& m, void* dst, const cx<T>* lds, int g0, int nt,
                             double scale) {
   const T s = (T)scale;
@@ -510,7 +533,7 @@ __device__ void store_stage(const KParams& P, const TMap& m, void* dst, const cx
       cx<T> v = row[lx(k)];
       if constexpr (ST == ST_PLAIN) {
         if (P.conj_out) v = cj(v);
-        out[k * es] = scl(v, s);
+        gput<NT>(&out[k * es], scl(v, s));
       } else if constexpr (ST == ST_TW) {
         if (P.twsh) {
           // two-level table: W^m = W^(m_hi << sh) * W^(m_lo), both rounded once
@@ -523,13 +546,13 @@ __device__ void store_stage(const KParams& P, const TMap& m, void* dst, const cx
           v = v * reinterpret_cast<const cx<T>*>(P.tab_a)[k * P.twk + c * P.twc];
         }
         if (P.conj_out) v = cj(v);
-        out[k * es] = scl(v, s);
+        gput<NT>(&out[k * es], scl(v, s));
       } else {
         const long long j = (long long)c * P.out.cstride + (long long)k * es;
         if (j < P.N) {
           v = cj(v) * reinterpret_cast<const cx<T>*>(P.chirp)[j];
           if (P.conj_out) v = cj(v);
-          reinterpret_cast<cx<T>*>(dst)[base - (long long)c * P.out.cstride + j] = scl(v, s);
+          gput<NT>(&reinterpret_cast<cx<T>*>(dst)[base - (long long)c * P.out.cstride + j], scl(v, s));
         }
       }
     }
@@ -539,7 +562,7 @@ __device__ void store_stage(const KParams& P, const TMap& m, void* dst, const cx
   if (t >= nt) return;
   const cx<T>* row = lds + t * P.Lp;
   const bool cjo = P.conj_out != 0;
-  post_ops<T, ST>(P, [&](int k) {
+  post_ops<T, ST, NT>(P, [&](int k) {
     const cx<T> z = row[lx(k)];
     return cjo ? cj(z) : z;
   }, dst, (long long)(g0 + t) * P.out.bdist, m.ur, m.tpt, s);

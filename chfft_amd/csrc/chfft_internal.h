@@ -118,6 +118,8 @@ struct KParams {
   const void* tab_a;
   const void* tab_b;
   int spec;         // multi-length kernel: register-resident tile path of this descriptor (0: generic)
+  int nt_out;       // 1: the launch writes the exec's final output (no later launch reads it): a registered
+                    // kernel with a non-temporal-output build runs that build
 };
 
 // One item of a multi-length batch (one transform). kind 0: fused (descs[desc]);

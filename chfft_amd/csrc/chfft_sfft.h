@@ -130,9 +130,9 @@ template <typename V>
 __device__ __forceinline__ V ld_off(const void* base, unsigned off) {
   return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + off);
 }
-template <typename V>
+template <typename V, bool NT = false>
 __device__ __forceinline__ void st_off(void* base, unsigned off, V v) {
-  *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + off) = v;
+  gput<NT>(reinterpret_cast<V*>(reinterpret_cast<char*>(base) + off), v);
 }
 
 // Padded LDS index of element base + j * S: linear in j when S is a multiple
@@ -340,15 +340,15 @@ __device__ __forceinline__ void run_all(cx<T>* x, int u, const cx<T>* __restrict
 }
 
 // ---------------------------------------------------------------- store operators (element-wise)
-template <typename T, int ST>
+template <typename T, int ST, bool NT = false>
 __device__ __forceinline__ void store_elem(const KParams& P, void* dst, long long obase, int c, int k, cx<T> v, T s) {
   if constexpr (ST == ST_PLAIN) {
     if (P.conj_out) v = cj(v);
-    reinterpret_cast<cx<T>*>(dst)[obase + (long long)k * P.out.estride] = scl(v, s);
+    gput<NT>(&reinterpret_cast<cx<T>*>(dst)[obase + (long long)k * P.out.estride], scl(v, s));
   } else if constexpr (ST == ST_TW) {
     v = v * reinterpret_cast<const cx<T>*>(P.tab_a)[k * P.twk + c * P.twc];
     if (P.conj_out) v = cj(v);
-    reinterpret_cast<cx<T>*>(dst)[obase + (long long)k * P.out.estride] = scl(v, s);
+    gput<NT>(&reinterpret_cast<cx<T>*>(dst)[obase + (long long)k * P.out.estride], scl(v, s));
   } else if constexpr (ST == ST_FS_DECHIRP) {
     // four-step Bluestein last pass: element k of column c is sequence index
     // c + k * estride; de-chirp (chirpz.rs:127-145) and crop to N
@@ -356,7 +356,7 @@ __device__ __forceinline__ void store_elem(const KParams& P, void* dst, long lon
     if (j < P.N) {
       v = cj(v) * reinterpret_cast<const cx<T>*>(P.chirp)[j];
       if (P.conj_out) v = cj(v);
-      reinterpret_cast<cx<T>*>(dst)[obase - (long long)c * P.out.cstride + j] = scl(v, s);
+      gput<NT>(&reinterpret_cast<cx<T>*>(dst)[obase - (long long)c * P.out.cstride + j], scl(v, s));
     }
   } else if constexpr (ST == ST_MDCT) {
     // mdct1d.rs:234-240
@@ -364,8 +364,8 @@ __device__ __forceinline__ void store_elem(const KParams& P, void* dst, long lon
     const int h = (int)(P.N >> 1);
     const cx<T> w = v * scl(reinterpret_cast<const cx<T>*>(P.tab_a)[k], s);
     T* ret = reinterpret_cast<T*>(dst) + obase;
-    ret[2 * k] = -w.re;
-    ret[h - 1 - 2 * k] = w.im;
+    gput<NT>(&ret[2 * k], -w.re);
+    gput<NT>(&ret[h - 1 - 2 * k], w.im);
   } else if constexpr (ST == ST_IMDCT) {
     // mdct1d.rs:268-288
     if (P.conj_out) v = cj(v);
@@ -374,14 +374,14 @@ __device__ __forceinline__ void store_elem(const KParams& P, void* dst, long lon
     const cx<T> z = v * scl(reinterpret_cast<const cx<T>*>(P.tab_a)[k], s);
     T* ret = reinterpret_cast<T*>(dst) + obase;
     const int mm = 2 * k;
-    ret[q3 - 1 - mm] = z.re * w[q3 - 1 - mm];
-    ret[q + mm] = -z.im * w[q + mm];
+    gput<NT>(&ret[q3 - 1 - mm], z.re * w[q3 - 1 - mm]);
+    gput<NT>(&ret[q + mm], -z.im * w[q + mm]);
     if (k < lo) {
-      ret[q3 + mm] = z.re * w[q3 + mm];
-      ret[q - 1 - mm] = z.im * w[q - 1 - mm];
+      gput<NT>(&ret[q3 + mm], z.re * w[q3 + mm]);
+      gput<NT>(&ret[q - 1 - mm], z.im * w[q - 1 - mm]);
     } else {
-      ret[mm - q] = -z.re * w[mm - q];
-      ret[q5 - 1 - mm] = -z.im * w[q5 - 1 - mm];
+      gput<NT>(&ret[mm - q], -z.re * w[mm - q]);
+      gput<NT>(&ret[q5 - 1 - mm], -z.im * w[q5 - 1 - mm]);
     }
   }
 }
@@ -393,7 +393,7 @@ __device__ __forceinline__ void store_elem(const KParams& P, void* dst, long lon
 // Natural-order store of a tile from its final LDS image (epilogue path) for
 // the element-wise operators PLAIN and TW; CM: consecutive threads on
 // consecutive transforms (transposed stores write whole segments).
-template <typename T, int L, int TPT, int TPB, int ST, bool CM>
+template <typename T, int L, int TPT, int TPB, int ST, bool CM, bool NT = false>
 __device__ __forceinline__ void epi_store(const KParams& P, void* dst, const cx<T>* lds, int LP, int g0, int nt,
                                           T s) {
   const int tid = threadIdx.x;
@@ -418,7 +418,7 @@ __device__ __forceinline__ void epi_store(const KParams& P, void* dst, const cx<
       cx<T> v = row[lxs<TPT>(u, r)];
       if constexpr (ST == ST_TW) v = v * ld_off<cx<T>>(reinterpret_cast<const char*>(P.tab_a) + (unsigned)(r * TPT) * tk, toff);
       if (cjo) v = cj(v);
-      st_off<cx<T>>(ob + (unsigned)(r * TPT) * es, voff, scl(v, s));
+      st_off<cx<T>, NT>(ob + (unsigned)(r * TPT) * es, voff, scl(v, s));
     }
   }
 }
@@ -427,7 +427,7 @@ __device__ __forceinline__ void epi_store(const KParams& P, void* dst, const cx<
 // tile of half length L (N = 2L): thread u of transform t takes the pairs
 // i = u + r * TPT, i <= L/2, and writes bins i, N-i, L-i and L+i; tables and
 // the row are addressed by 32-bit offsets.
-template <typename T, int L, int TPT, int TPB>
+template <typename T, int L, int TPT, int TPB, bool NT = false>
 __device__ __forceinline__ void epi_dct2(const KParams& P, void* dst, const cx<T>* lds, int LP, int g0, int nt, T s) {
   constexpr int N = 2 * L, h = L, q2 = L / 2;  // N % 4 == 0 when L is even
   const int tid = threadIdx.x;
@@ -458,10 +458,10 @@ __device__ __forceinline__ void epi_dct2(const KParams& P, void* dst, const cx<T
         const cx<T> x = ld_off<cx<T>>(co, (unsigned)i * CE) * (zi - wc);
         const cx<T> jj = cj(zi - x) * ld_off<cx<T>>(om, (unsigned)i * CE);
         const cx<T> kk = (wc + x) * ld_off<cx<T>>(om, (unsigned)(h - i) * CE);
-        st_off<T>(ret, (unsigned)i * RE, jj.re * s);
-        st_off<T>(ret, (unsigned)(N - i) * RE, jj.im * s);
-        st_off<T>(ret, (unsigned)(h - i) * RE, kk.re * s);
-        st_off<T>(ret, (unsigned)(h + i) * RE, kk.im * s);
+        st_off<T, NT>(ret, (unsigned)i * RE, jj.re * s);
+        st_off<T, NT>(ret, (unsigned)(N - i) * RE, jj.im * s);
+        st_off<T, NT>(ret, (unsigned)(h - i) * RE, kk.re * s);
+        st_off<T, NT>(ret, (unsigned)(h + i) * RE, kk.im * s);
       } else {  // i == q2
         const cx<T> x = Z(q2) * om[q2];
         ret[q2] = x.re * s;
@@ -474,7 +474,7 @@ __device__ __forceinline__ void epi_dct2(const KParams& P, void* dst, const cx<T
 // DCT-III output (dct1d.rs:395-407) from the final LDS image, half length L
 // even: pair i < L/2 writes ret[4i .. 4i+3] = (a.re, b.im, a.im, b.re) with
 // a = Z(i), b = Z(L-1-i), as two aligned two-real stores.
-template <typename T, int L, int TPT, int TPB>
+template <typename T, int L, int TPT, int TPB, bool NT = false>
 __device__ __forceinline__ void epi_dct3(const KParams& P, void* dst, const cx<T>* lds, int LP, int g0, int nt, T s) {
   constexpr int h = L, q2 = L / 2;
   const int tid = threadIdx.x;
@@ -494,8 +494,8 @@ __device__ __forceinline__ void epi_dct3(const KParams& P, void* dst, const cx<T
         a = cj(a);
         b = cj(b);
       }
-      st_off<cx<T>>(ret, (unsigned)(4 * i) * RE, mk(a.re * s, b.im * s));
-      st_off<cx<T>>(ret, (unsigned)(4 * i + 2) * RE, mk(a.im * s, b.re * s));
+      st_off<cx<T>, NT>(ret, (unsigned)(4 * i) * RE, mk(a.re * s, b.im * s));
+      st_off<cx<T>, NT>(ret, (unsigned)(4 * i + 2) * RE, mk(a.im * s, b.re * s));
     }
   }
 }
@@ -819,7 +819,7 @@ struct StoreTw {
 // MODE: 0 plain passes; 1 fused Bluestein (chirp on load, spectrum product,
 // second pass sequence, de-chirp); 2 spectrum product + second sequence only
 // (the row pass of a four-step Bluestein).
-template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS, int AL>
+template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS, int AL, bool NT>
 __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, void* dst, int g0, double scale,
                                           cx<T>* lds, const StoreTw<T, Plan<RS, L, TPT>, TPT, ST, EPI>& stw) {
   constexpr bool BLUE = MODE == 1;
@@ -933,22 +933,22 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
     const int nt = min(TPB, P.G - g0);
     if constexpr (ST == ST_PLAIN || ST == ST_TW) {
       if (P.colmajor_st)
-        epi_store<T, L, TPT, TPB, ST, true>(P, dst, img, LP, g0, nt, (T)scale);
+        epi_store<T, L, TPT, TPB, ST, true, NT>(P, dst, img, LP, g0, nt, (T)scale);
       else
-        epi_store<T, L, TPT, TPB, ST, false>(P, dst, img, LP, g0, nt, (T)scale);
+        epi_store<T, L, TPT, TPB, ST, false, NT>(P, dst, img, LP, g0, nt, (T)scale);
     } else if constexpr (ST == ST_DCT2 && L % 2 == 0) {
-      epi_dct2<T, L, TPT, TPB>(P, dst, img, LP, g0, nt, (T)scale);
+      epi_dct2<T, L, TPT, TPB, NT>(P, dst, img, LP, g0, nt, (T)scale);
     } else if constexpr (ST == ST_DCT3 && L % 2 == 0) {
       if (pair_aligned<AL>(dst, P.out.bdist, sizeof(cx<T>))) {
-        epi_dct3<T, L, TPT, TPB>(P, dst, img, LP, g0, nt, (T)scale);
+        epi_dct3<T, L, TPT, TPB, NT>(P, dst, img, LP, g0, nt, (T)scale);
       } else {
         const TMap m = make_map(TPB);
-        store_stage<T, ST>(P, m, dst, img, g0, nt, scale);
+        store_stage<T, ST, NT>(P, m, dst, img, g0, nt, scale);
       }
     } else {
       // the planner sets P.Lp == LP for specialised launches
       const TMap m = make_map(TPB);
-      store_stage<T, ST>(P, m, dst, img, g0, nt, scale);
+      store_stage<T, ST, NT>(P, m, dst, img, g0, nt, scale);
     }
   } else {
     if (!valid) return;
@@ -982,7 +982,7 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
                 v = v * (ta[m >> sh] * tb[m & lo]);
               }
               if (cjo) v = cj(v);
-              st_off<cx<T>>(ob + (unsigned)rel * es, voff, scl(v, s));
+              st_off<cx<T>, NT>(ob + (unsigned)rel * es, voff, scl(v, s));
             }
           }
         }
@@ -1021,7 +1021,7 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
           for (int j = 0; j < RL; j++) {
             const int k = u + b * TPT + j * PLAST;
             const T pim = __shfl_xor(wim[(BL - 1 - b) * RL + (RL - 1 - j)], TPT - 1);
-            st_off<cx<T>>(ret, (unsigned)(2 * k) * RE, mk(-wre[b * RL + j], pim));
+            st_off<cx<T>, NT>(ret, (unsigned)(2 * k) * RE, mk(-wre[b * RL + j], pim));
           }
       } else {
         const int h = (int)(P.N >> 1);
@@ -1051,7 +1051,7 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
 #pragma unroll
           for (int j = 0; j < RL; j++) {
             const int k = i + j * PLAST;
-            if (!BLUE || (lower_half<RL>(j) && k < P.L)) store_elem<T, ST>(P, dst, obase, c, k, x[b * RL + j], s);
+            if (!BLUE || (lower_half<RL>(j) && k < P.L)) store_elem<T, ST, NT>(P, dst, obase, c, k, x[b * RL + j], s);
           }
         }
       }
@@ -1059,7 +1059,8 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
   }
 }
 
-template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS, int AL = 2>
+template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS, int AL = 2,
+          bool NT = false>
 __device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, void* dst, int g0, double scale,
                                           cx<T>* lds) {
   StoreTw<T, Plan<RS, L, TPT>, TPT, ST, EPI> stw;
@@ -1071,7 +1072,7 @@ __device__ __forceinline__ void sfft_tile(const KParams& P, const void* src, voi
     io_base(P.out, min(g0 + t, P.G - 1), &c);
     stw.load(P, u, c);
   }
-  sfft_body<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, RS, AL>(P, src, dst, g0, scale, lds, stw);
+  sfft_body<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, RS, AL, NT>(P, src, dst, g0, scale, lds, stw);
 }
 
 // Tile index of this workgroup, arranged so that the workgroups one XCD runs

@@ -6,12 +6,13 @@
 
 namespace chfft {
 
-template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS, int AL>
+template <typename T, int L, int TPT, int TPB, bool COL, int LD, int ST, int MODE, bool EPI, class RS, int AL,
+          bool NT = false>
 __global__ void __launch_bounds__(TPT * TPB) sfft_kernel(KParams P) {
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem);
   const int tile = (COL || P.colmajor_st) ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-  sfft_tile<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, RS, AL>(P, P.src, P.dst, tile * TPB, P.scale, lds);
+  sfft_tile<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, RS, AL, NT>(P, P.src, P.dst, tile * TPB, P.scale, lds);
 }
 
 // Persistent four-step transform with length-specialised phases: one
@@ -82,6 +83,8 @@ struct SfftEntry {
   int rad[8];
   void (*fn)(KParams);   // the kernel (paired-real operators: aligned buffers)
   void (*fn0)(KParams);  // paired-real operators on unaligned buffers (else fn)
+  void (*fnt)(KParams);  // fn with non-temporal output stores (null: none), for launches that write the
+                         // exec's final output
 };
 
 // Operators with paired-real fast paths get two kernels: one that assumes the
@@ -94,28 +97,39 @@ constexpr int sf_al(int ld, int st, bool aligned) { return sf_paired(ld, st) ? (
 
 #define SF_KERNEL(T, L, TPT, TPB, COL, LD, ST, MODE, EPI, AL, ...) \
   &sfft_kernel<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, Rad<__VA_ARGS__>, AL>
+#define SF_KERNEL_NT(T, L, TPT, TPB, COL, LD, ST, MODE, EPI, AL, ...) \
+  &sfft_kernel<T, L, TPT, TPB, COL, LD, ST, MODE, EPI, Rad<__VA_ARGS__>, AL, true>
 
-#define SF_ENTRY(F64, T, L, TPT, TPB, COL, LD, ST, MODE, EPI, NR, ...)                                      \
+#define SF_ENTRY_(NTK, F64, T, L, TPT, TPB, COL, LD, ST, MODE, EPI, NR, ...)                                \
   {F64, L, TPT, TPB, COL, LD, ST, MODE, EPI, NR, {__VA_ARGS__},                                             \
    reinterpret_cast<void (*)(KParams)>(SF_KERNEL(T, L, TPT, TPB, COL, LD, ST, MODE, EPI, sf_al(LD, ST, true), \
                                                  __VA_ARGS__)),                                             \
    reinterpret_cast<void (*)(KParams)>(SF_KERNEL(T, L, TPT, TPB, COL, LD, ST, MODE, EPI, sf_al(LD, ST, false), \
-                                                 __VA_ARGS__))}
+                                                 __VA_ARGS__)),                                             \
+   NTK(T, L, TPT, TPB, COL, LD, ST, MODE, EPI, sf_al(LD, ST, true), __VA_ARGS__)}
+#define SF_NO_NT(...) nullptr
+#define SF_WITH_NT(...) reinterpret_cast<void (*)(KParams)>(SF_KERNEL_NT(__VA_ARGS__))
+#define SF_ENTRY(...) SF_ENTRY_(SF_NO_NT, __VA_ARGS__)
+#define SF_ENTRY_NT(...) SF_ENTRY_(SF_WITH_NT, __VA_ARGS__)
 
 #define SF_BOTH(L, TPT, TPB, TPB64, COL, LD, ST, MODE, EPI, NR, ...)                    \
   SF_ENTRY(0, float, L, TPT, TPB, COL, LD, ST, MODE, EPI, NR, __VA_ARGS__),              \
       SF_ENTRY(1, double, L, TPT, TPB64, COL, LD, ST, MODE, EPI, NR, __VA_ARGS__)
+// the same with a non-temporal-output build of each kernel
+#define SF_BOTH_NT(L, TPT, TPB, TPB64, COL, LD, ST, MODE, EPI, NR, ...)                 \
+  SF_ENTRY_NT(0, float, L, TPT, TPB, COL, LD, ST, MODE, EPI, NR, __VA_ARGS__),           \
+      SF_ENTRY_NT(1, double, L, TPT, TPB64, COL, LD, ST, MODE, EPI, NR, __VA_ARGS__)
 
 static const SfftEntry kSfft[] = {
     // 4096: CFft1D rows, RFft1D N=8192 forward / backward
-    SF_BOTH(4096, 256, 1, 1, false, LD_PLAIN, ST_PLAIN, 0, false, 3, 16, 16, 16),
-    SF_BOTH(4096, 256, 1, 1, false, LD_PLAIN, ST_R2C, 0, true, 3, 16, 16, 16),
-    SF_BOTH(4096, 256, 1, 1, false, LD_R2C_BACK, ST_PLAIN, 0, false, 3, 16, 16, 16),
+    SF_BOTH_NT(4096, 256, 1, 1, false, LD_PLAIN, ST_PLAIN, 0, false, 3, 16, 16, 16),
+    SF_BOTH_NT(4096, 256, 1, 1, false, LD_PLAIN, ST_R2C, 0, true, 3, 16, 16, 16),
+    SF_BOTH_NT(4096, 256, 1, 1, false, LD_R2C_BACK, ST_PLAIN, 0, false, 3, 16, 16, 16),
     // 512: Dct1D / Mdct1D N=1024 / 2048 inner transforms, and CFft1D
     SF_BOTH(512, 32, 8, 4, false, LD_PLAIN, ST_PLAIN, 0, false, 3, 16, 8, 4),
     SF_BOTH(512, 32, 8, 4, false, LD_DCT2, ST_DCT2, 0, true, 3, 16, 8, 4),
     SF_BOTH(512, 32, 8, 4, false, LD_DCT3, ST_DCT3, 0, true, 3, 16, 8, 4),
-    SF_BOTH(512, 32, 8, 4, false, LD_MDCT, ST_MDCT, 0, false, 3, 16, 8, 4),
+    SF_BOTH_NT(512, 32, 8, 4, false, LD_MDCT, ST_MDCT, 0, false, 3, 16, 8, 4),
     SF_BOTH(512, 32, 8, 4, false, LD_IMDCT, ST_IMDCT, 0, false, 3, 16, 8, 4),
     // 2048: CFft2D rows, CFft1D
     SF_BOTH(2048, 128, 2, 1, false, LD_PLAIN, ST_PLAIN, 0, false, 3, 16, 16, 8),
@@ -123,7 +137,7 @@ static const SfftEntry kSfft[] = {
     // 8 (FP32) / 4 (FP64) columns, one 128 KB tile per CU
     SF_BOTH(2048, 128, 8, 4, true, LD_PLAIN, ST_PLAIN, 0, false, 3, 16, 16, 8),
     // 8192: fused Bluestein for N <= 4096 (N = 4093 and smaller primes), CFft1D
-    SF_BOTH(8192, 512, 1, 1, false, LD_PLAIN, ST_PLAIN, 1, false, 4, 16, 16, 8, 4),
+    SF_BOTH_NT(8192, 512, 1, 1, false, LD_PLAIN, ST_PLAIN, 1, false, 4, 16, 16, 8, 4),
     SF_BOTH(8192, 512, 1, 1, false, LD_PLAIN, ST_PLAIN, 0, false, 4, 16, 16, 8, 4),
     // 20160 = 2^6 3^2 5 7 (FP32): fused Bluestein for 8192 < N <= 10080 (N =
     // 10007), the whole padded sequence (20475 slots) in one CU's LDS; 768
@@ -131,7 +145,7 @@ static const SfftEntry kSfft[] = {
     // of the spans 16 and 192 so that those passes' twiddles repeat across a
     // thread's butterflies (radix 15 first, whose stride-15 first-pass writes
     // spread over the LDS banks, measured 94 us against 87)
-    SF_ENTRY(0, float, 20160, 768, 1, false, LD_PLAIN, ST_PLAIN, 1, false, 4, 16, 12, 15, 7),
+    SF_ENTRY_NT(0, float, 20160, 768, 1, false, LD_PLAIN, ST_PLAIN, 1, false, 4, 16, 12, 15, 7),
     // 43200 = 180 x 240 (rows of the column pass 128-byte aligned), two
     // passes each: 180 = 12 x 15, 240 = 16 x 15 (prime-factor radices)
     SF_BOTH(180, 16, 16, 8, true, LD_PLAIN, ST_TW, 0, false, 2, 12, 15),
@@ -265,12 +279,15 @@ bool sfft_has_schedule(bool f64, int L, const int* rad, int nrad) {
 
 hipError_t sfft_launch(const SfftEntry* e, const KParams& P, dim3 grid, size_t lds, hipStream_t st) {
   void (*fn)(KParams) = e->fn;
+  bool al = true;
   if (e->fn0 != e->fn) {
     const size_t esz = e->f64 ? 16 : 8;
-    const bool al = (!(e->ld == LD_DCT2 || e->ld == LD_MDCT) || pair_aligned<2>(P.src, P.in.bdist, esz)) &&
-                    (!(e->st == ST_DCT3 || e->st == ST_MDCT) || pair_aligned<2>(P.dst, P.out.bdist, esz));
+    al = (!(e->ld == LD_DCT2 || e->ld == LD_MDCT) || pair_aligned<2>(P.src, P.in.bdist, esz)) &&
+         (!(e->st == ST_DCT3 || e->st == ST_MDCT) || pair_aligned<2>(P.dst, P.out.bdist, esz));
     if (!al) fn = e->fn0;
   }
+  // the non-temporal build runs the aligned fast paths only
+  if (P.nt_out && e->fnt && al) fn = e->fnt;
   hipLaunchKernelGGL(fn, grid, dim3(e->tpt * e->tpb), lds, st, P);
   return hipGetLastError();
 }
@@ -282,7 +299,8 @@ hipError_t sfft_set_lds_limits() {
     if (x != hipSuccess) r = x;
   }
   for (const SfftEntry& e : kSfft)
-    for (void (*fn)(KParams) : {e.fn, e.fn0}) {
+    for (void (*fn)(KParams) : {e.fn, e.fn0, e.fnt}) {
+      if (!fn) continue;
       hipError_t x = hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)kLdsBytes);
       if (x != hipSuccess) r = x;
