@@ -117,6 +117,11 @@ struct KParams {
   const void* kspec;
   const void* tab_a;
   const void* tab_b;
+  // four-step twiddles by recurrence (length-specialised ST_TW stores): FP64
+  // table [r][c] of W^(c r) for r < K0 (the kernel's base rows, K0 = its
+  // last-pass butterflies per thread x threads per transform) and, in row K0,
+  // the step W^(c p) (p: the last pass's span); null: the full table tab_a
+  const void* tw_rec;
   int spec;         // multi-length kernel: register-resident tile path of this descriptor (0: generic)
   int nt_out;       // 1: the launch writes the exec's final output (no later launch reads it): a registered
                     // kernel with a non-temporal-output build runs that build

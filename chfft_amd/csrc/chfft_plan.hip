@@ -829,6 +829,26 @@ IoDesc io(long long bdist, long long cstride, long long estride, int ncols) {
 
 }  // namespace
 
+// Twiddle recurrence table of a registered four-step store kernel (ST_TW):
+// the kernel's thread u of a transform holds outputs k = u + b TPT + j p
+// (b < BL, j < RL; RL the last radix, p = L / RL its span, BL TPT >= the
+// last pass's butterflies) and forms W_Ntw^(c k) from the base row u + b TPT
+// and the step row K0 = BL TPT of this table, in FP64 (each twiddle rounded
+// once to the transform's precision): one table read per row of outputs
+// instead of one per output. Layout [r][c], c < ncols.
+int upload_tw_recurrence(const SfftEntry* e, size_t Ntw, size_t ncols, DevBuf& dst) {
+  const int rl = e->rad[e->nrad - 1];
+  const int span = e->L / rl;  // also the last pass's butterfly count
+  const int K0 = (span + e->tpt - 1) / e->tpt * e->tpt;
+  std::vector<ld_t> v(2 * (size_t)(K0 + 1) * ncols);
+  for (int r = 0; r <= K0; r++) {
+    const unsigned long long m = r < K0 ? (unsigned long long)r : (unsigned long long)span;
+    for (size_t c = 0; c < ncols; c++)
+      unit_root(m * c, Ntw, &v[2 * ((size_t)r * ncols + c)], &v[2 * ((size_t)r * ncols + c) + 1]);
+  }
+  return upload(v, true, dst);
+}
+
 // Complex-to-complex engine for one length (used by CFft1D, both CFft2D axes).
 struct C2CEngine {
   size_t n = 0;
@@ -839,6 +859,7 @@ struct C2CEngine {
   SeqDesc s1{}, s2{};
   SeqDesc x1{}, x2{};  // prime-factor schedules of n1, n2 (L = 0: none registered)
   DevBuf tw1, tw2, twx1, twx2, twN, chirp, kspec;
+  DevBuf twr_a, twr_b;  // twiddle recurrence tables of the column / Bluestein row store kernels
   size_t M = 0;
   Tune tune;  // planner overrides (the autotuner's candidates)
 
@@ -891,6 +912,12 @@ struct C2CEngine {
     if ((rc = build_pfa(n1, x1, twx1))) return rc;
     if ((rc = build_pfa(n2, x2, twx2))) return rc;
     if ((rc = upload(fourstep_table(n1, n2), f64, twN))) return rc;
+    if (!tune.generic) {
+      const SfftEntry* ea = sfft_find(f64, (int)n1, mode == 2 ? LD_PLAIN : LD_FS_CHIRP, ST_TW, 0, true);
+      if (ea && (rc = upload_tw_recurrence(ea, total, n2, twr_a))) return rc;
+      const SfftEntry* eb = mode == 3 ? sfft_find(f64, (int)n2, LD_PLAIN, ST_TW, 2, false) : nullptr;
+      if (eb && (rc = upload_tw_recurrence(eb, total, n1, twr_b))) return rc;
+    }
     if (mode == 3) {
       if ((rc = upload(chirp_table(len), f64, chirp))) return rc;
       // the row pass reads the spectrum at bins k1 + n1 * k2 (row k1, element
@@ -1033,6 +1060,7 @@ struct C2CEngine {
       finish(a, choose_geometry((int)n1, a.P.G, true, f64), src, S_SCRATCH, 0);
       a.what = "four-step columns " + seq_str(s1) + " + twiddle";
       if (!tune.generic) try_specialise(a, f64, &x1);
+      if (a.kind == 3 && twr_a.p) a.P.tw_rec = twr_a.p;
     out.push_back(a);
       // K2: rows (length n2) -> out[k1 + n1*k2]
       Launch b{};
@@ -1069,6 +1097,7 @@ struct C2CEngine {
     finish(a, choose_geometry((int)n1, a.P.G, true, f64), src, S_SCRATCH, 0);
     a.what = "bluestein N=" + std::to_string(n) + " chirp + columns " + seq_str(s1) + " + twiddle";
     if (!tune.generic) try_specialise(a, f64, &x1);
+    if (a.kind == 3 && twr_a.p) a.P.tw_rec = twr_a.p;
     out.push_back(a);
     Launch b{};
     b.ld = LD_PLAIN;
@@ -1088,6 +1117,7 @@ struct C2CEngine {
     finish(b, choose_geometry((int)n2, b.P.G, false, f64), S_SCRATCH, S_SCRATCH, 0);
     b.what = "bluestein rows " + seq_str(s2) + " x spectrum, inverse rows + twiddle";
     if (!tune.generic) try_specialise(b, f64, &x2);
+    if (b.kind == 3 && twr_b.p) b.P.tw_rec = twr_b.p;
     out.push_back(b);
     Launch c{};
     c.ld = LD_PLAIN;

@@ -800,6 +800,26 @@ struct StoreTw {
   cx<T> w[EARLY ? BL * RL : 1];
   __device__ __forceinline__ void load(const KParams& P, int u, int c) {
     if constexpr (EARLY) {
+      if (P.tw_rec) {
+        // W^(c k) for k = u + b TPT + j PLAST: base W^(c (u + b TPT)) times
+        // j steps W^(c PLAST), in FP64, each result rounded once
+        constexpr int K0 = BL * TPT;
+        const cx<double>* tr = reinterpret_cast<const cx<double>*>(P.tw_rec);
+        const int nc = P.out.ncols;
+        const cx<double> st = tr[K0 * nc + c];
+#pragma unroll
+        for (int b = 0; b < BL; b++) {
+          if (PL::FULL(NP - 1) || u + b * TPT < PL::NB(NP - 1)) {
+            cx<double> wd = tr[(u + b * TPT) * nc + c];
+#pragma unroll
+            for (int j = 0; j < RL; j++) {
+              w[b * RL + j] = mk((T)wd.re, (T)wd.im);
+              if (j + 1 < RL) wd = wd * st;
+            }
+          }
+        }
+        return;
+      }
       constexpr unsigned ESZ = sizeof(cx<T>);
       const unsigned tk = (unsigned)P.twk * ESZ;
       const unsigned toff = ((unsigned)u * (unsigned)P.twk + (unsigned)c * (unsigned)P.twc) * ESZ;
@@ -964,10 +984,18 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
       const cx<T>* tb = reinterpret_cast<const cx<T>*>(P.tab_b);
       const unsigned sh = (unsigned)P.twsh, lo = (1u << sh) - 1u;
       const unsigned mu = (unsigned)co * (unsigned)u;
+      // twiddles by recurrence (see StoreTw::load) where the plan has the table
+      constexpr bool REC = ST == ST_TW && !TW_EARLY;
+      const cx<double>* trc = REC ? reinterpret_cast<const cx<double>*>(P.tw_rec) : nullptr;
+      const int trn = P.out.ncols;
+      cx<double> wst = mk(1.0, 0.0);
+      if (REC && trc) wst = trc[BL * TPT * trn + co];
 #pragma unroll
       for (int b = 0; b < BL; b++) {
         const int i = u + b * TPT;
         if (PL::FULL(NP - 1) || i < PL::NB(NP - 1)) {
+          cx<double> wd = mk(1.0, 0.0);
+          if (REC && trc) wd = trc[i * trn + co];
 #pragma unroll
           for (int j = 0; j < RL; j++) {
             const int rel = b * TPT + j * PLAST;  // k = u + rel
@@ -975,15 +1003,19 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
               cx<T> v = x[b * RL + j];
               if constexpr (TW_EARLY)
                 v = v * stw.w[b * RL + j];
-              else if constexpr (ST == ST_TW)
-                v = v * ld_off<cx<T>>(reinterpret_cast<const char*>(P.tab_a) + (unsigned)rel * tk, toff);
-              else if constexpr (ST == ST_TW2) {
+              else if constexpr (ST == ST_TW) {
+                if (trc)
+                  v = v * mk((T)wd.re, (T)wd.im);
+                else
+                  v = v * ld_off<cx<T>>(reinterpret_cast<const char*>(P.tab_a) + (unsigned)rel * tk, toff);
+              } else if constexpr (ST == ST_TW2) {
                 const unsigned m = mu + (unsigned)co * (unsigned)rel;
                 v = v * (ta[m >> sh] * tb[m & lo]);
               }
               if (cjo) v = cj(v);
               st_off<cx<T>, NT>(ob + (unsigned)rel * es, voff, scl(v, s));
             }
+            if (REC && trc && j + 1 < RL) wd = wd * wst;
           }
         }
       }
