@@ -16,6 +16,19 @@ struct alignas(2 * sizeof(T)) cx {
   T re, im;
 };
 
+// A pointer the caller knows to address global memory, as a global-address-
+// space pointer, so that accesses through it compile to global instructions
+// rather than flat ones. Pointers read from descriptors in device memory
+// (the multi-length kernel's) are otherwise flat, and a flat access also
+// counts against the LDS wait counter: the LDS-only wait before each barrier
+// would drain the global loads in flight. The integer round trip keeps the
+// compiler from folding the address-space casts away.
+template <typename T>
+__device__ __forceinline__ T* as_global(T* p) {
+  typedef __attribute__((address_space(1))) T G;
+  return (T*)(G*)(uintptr_t)p;
+}
+
 // Store of one output value to global memory; NT: non-temporal (the L2
 // streams the line out instead of keeping it), for the exec's final output
 // in the kernels compiled for it.
@@ -325,8 +338,8 @@ __device__ __forceinline__ Loader<T, LD> make_loader(const KParams& P, const voi
   Loader<T, LD> L;
   int c;
   const long long base = io_base(P.in, g, &c);
-  L.cs = reinterpret_cast<const cx<T>*>(src) + base;
-  L.rs = reinterpret_cast<const T*>(src) + base;
+  L.cs = reinterpret_cast<const cx<T>*>(as_global(src)) + base;
+  L.rs = reinterpret_cast<const T*>(as_global(src)) + base;
   L.seq0 = (long long)c * P.in.cstride;
   L.brow = base - L.seq0;
   return L;
@@ -343,11 +356,11 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, const Loader<T, LD>
     if (s >= P.N) return zero;
     cx<T> v = reinterpret_cast<const cx<T>*>(ld.cs - ld.seq0)[s];
     if (P.conj_in) v = cj(v);  // inverse: conjugate the input, not the chirp
-    return v * reinterpret_cast<const cx<T>*>(P.chirp)[s];
+    return v * reinterpret_cast<const cx<T>*>(as_global(P.chirp))[s];
   } else if constexpr (LD == LD_R2C_BACK) {
     // rfft1d.rs:161-177 (complex input, N/2+1 bins per row)
     const cx<T>* X = ld.cs;
-    const cx<T>* bc = reinterpret_cast<const cx<T>*>(P.tab_a);
+    const cx<T>* bc = reinterpret_cast<const cx<T>*>(as_global(P.tab_a));
     const int N = (int)P.N, h = N >> 1, q = (N + 3) >> 2;
     if (n == 0) {
       const T a = X[0].re, z = X[h].re;
@@ -372,8 +385,8 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, const Loader<T, LD>
   } else if constexpr (LD == LD_DCT3) {
     // dct1d.rs:367-391 (real input)
     const T* x = ld.rs;
-    const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
-    const cx<T>* om = reinterpret_cast<const cx<T>*>(P.tab_b);
+    const cx<T>* co = reinterpret_cast<const cx<T>*>(as_global(P.tab_a));
+    const cx<T>* om = reinterpret_cast<const cx<T>*>(as_global(P.tab_b));
     const int N = (int)P.N, h = N >> 1, q = (N + 3) >> 2;
     if (n == 0) {
       const T z = x[0] * (T)P.ortho;
@@ -389,8 +402,8 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, const Loader<T, LD>
   } else if constexpr (LD == LD_MDCT) {
     // mdct1d.rs:206-230 (frames `bdist` samples apart)
     const T* x = ld.rs;
-    const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
-    const T* w = reinterpret_cast<const T*>(P.tab_b);
+    const cx<T>* tw = reinterpret_cast<const cx<T>*>(as_global(P.tab_a));
+    const T* w = reinterpret_cast<const T*>(as_global(P.tab_b));
     const int N = (int)P.N, q = N >> 2, h = N >> 1, q3 = h + q, q5 = h + q3, lo = q - (q >> 1);
     const int m = n << 1;
     cx<T> v;
@@ -401,7 +414,7 @@ __device__ __forceinline__ cx<T> load_elem(const KParams& P, const Loader<T, LD>
     return v * tw[n];
   } else {  // LD_IMDCT, mdct1d.rs:261-265
     const T* c = ld.rs;
-    const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
+    const cx<T>* tw = reinterpret_cast<const cx<T>*>(as_global(P.tab_a));
     const int h = (int)(P.N >> 1);
     return mk(c[2 * n], c[h - 1 - 2 * n]) * scl(tw[n], T(-2));
   }
@@ -420,8 +433,8 @@ __device__ __forceinline__ void post_ops(const KParams& P, ZF Z, void* dst, long
     // come from the same pair Z(i), Z(h - i) and twiddle co[i]: one thread
     // computes both; i = 0 gives bins 0 and h, i = q (N % 4 == 0) bin q.
     const int N = (int)P.N, h = N >> 1, q = (N + 3) >> 2;
-    const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
-    cx<T>* out = reinterpret_cast<cx<T>*>(dst) + obase;
+    const cx<T>* co = reinterpret_cast<const cx<T>*>(as_global(P.tab_a));
+    cx<T>* out = reinterpret_cast<cx<T>*>(as_global(dst)) + obase;
     for (int i = i0; i <= q; i += step) {
       if (i == 0) {
         const cx<T> z = Z(0);
@@ -439,9 +452,9 @@ __device__ __forceinline__ void post_ops(const KParams& P, ZF Z, void* dst, long
   } else if constexpr (ST == ST_DCT2) {
     // dct1d.rs:266-287
     const int N = (int)P.N, h = N >> 1, q2 = (N + 3) >> 2;
-    const cx<T>* co = reinterpret_cast<const cx<T>*>(P.tab_a);
-    const cx<T>* om = reinterpret_cast<const cx<T>*>(P.tab_b);
-    T* ret = reinterpret_cast<T*>(dst) + obase;
+    const cx<T>* co = reinterpret_cast<const cx<T>*>(as_global(P.tab_a));
+    const cx<T>* om = reinterpret_cast<const cx<T>*>(as_global(P.tab_b));
+    T* ret = reinterpret_cast<T*>(as_global(dst)) + obase;
     for (int i = i0; i <= q2; i += step) {
       if (i == 0) {
         const cx<T> z = Z(0);
@@ -467,7 +480,7 @@ __device__ __forceinline__ void post_ops(const KParams& P, ZF Z, void* dst, long
     // dct1d.rs:395-407
     const int N = (int)P.N, h = N >> 1, q2 = h >> 1;
     const int per = q2 + (h & 1);
-    T* ret = reinterpret_cast<T*>(dst) + obase;
+    T* ret = reinterpret_cast<T*>(as_global(dst)) + obase;
     for (int i = i0; i < per; i += step) {
       const cx<T> a = Z(i);
       if (i < q2) {
@@ -485,8 +498,8 @@ __device__ __forceinline__ void post_ops(const KParams& P, ZF Z, void* dst, long
   } else if constexpr (ST == ST_MDCT) {
     // mdct1d.rs:234-240
     const int N = (int)P.N, q = N >> 2, h = N >> 1;
-    const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
-    T* ret = reinterpret_cast<T*>(dst) + obase;
+    const cx<T>* tw = reinterpret_cast<const cx<T>*>(as_global(P.tab_a));
+    T* ret = reinterpret_cast<T*>(as_global(dst)) + obase;
     for (int i = i0; i < q; i += step) {
       const cx<T> v = Z(i) * scl(tw[i], s);
       gput<NT>(&ret[2 * i], -v.re);
@@ -494,9 +507,9 @@ __device__ __forceinline__ void post_ops(const KParams& P, ZF Z, void* dst, long
     }
   } else {  // ST_IMDCT, mdct1d.rs:268-288
     const int N = (int)P.N, q = N >> 2, h = N >> 1, q3 = h + q, q5 = h + q3, lo = q - (q >> 1);
-    const cx<T>* tw = reinterpret_cast<const cx<T>*>(P.tab_a);
-    const T* w = reinterpret_cast<const T*>(P.tab_b);
-    T* ret = reinterpret_cast<T*>(dst) + obase;
+    const cx<T>* tw = reinterpret_cast<const cx<T>*>(as_global(P.tab_a));
+    const T* w = reinterpret_cast<const T*>(as_global(P.tab_b));
+    T* ret = reinterpret_cast<T*>(as_global(dst)) + obase;
     for (int i = i0; i < q; i += step) {
       const cx<T> v = Z(i) * scl(tw[i], s);
       const int mm = 2 * i;
@@ -527,7 +540,7 @@ __device__ void store_stage(const KParams& P, const TMap& m, void* dst, const cx
     const cx<T>* row = lds + t * P.Lp;
     int c;
     const long long base = io_base(P.out, g0 + t, &c);
-    cx<T>* out = reinterpret_cast<cx<T>*>(dst) + base;
+    cx<T>* out = reinterpret_cast<cx<T>*>(as_global(dst)) + base;
     const long long es = P.out.estride;
     for (int k = u; k < P.L; k += m.tpt) {
       cx<T> v = row[lx(k)];
@@ -538,21 +551,21 @@ __device__ void store_stage(const KParams& P, const TMap& m, void* dst, const cx
         if (P.twsh) {
           // two-level table: W^m = W^(m_hi << sh) * W^(m_lo), both rounded once
           const int mm = c * k;
-          v = v * (reinterpret_cast<const cx<T>*>(P.tab_a)[mm >> P.twsh] *
-                   reinterpret_cast<const cx<T>*>(P.tab_b)[mm & ((1 << P.twsh) - 1)]);
+          v = v * (reinterpret_cast<const cx<T>*>(as_global(P.tab_a))[mm >> P.twsh] *
+                   reinterpret_cast<const cx<T>*>(as_global(P.tab_b))[mm & ((1 << P.twsh) - 1)]);
         } else {
           // table of W^(c*k) laid out so that consecutive lanes read
           // consecutive entries: index k * twk + c * twc
-          v = v * reinterpret_cast<const cx<T>*>(P.tab_a)[k * P.twk + c * P.twc];
+          v = v * reinterpret_cast<const cx<T>*>(as_global(P.tab_a))[k * P.twk + c * P.twc];
         }
         if (P.conj_out) v = cj(v);
         gput<NT>(&out[k * es], scl(v, s));
       } else {
         const long long j = (long long)c * P.out.cstride + (long long)k * es;
         if (j < P.N) {
-          v = cj(v) * reinterpret_cast<const cx<T>*>(P.chirp)[j];
+          v = cj(v) * reinterpret_cast<const cx<T>*>(as_global(P.chirp))[j];
           if (P.conj_out) v = cj(v);
-          gput<NT>(&reinterpret_cast<cx<T>*>(dst)[base - (long long)c * P.out.cstride + j], scl(v, s));
+          gput<NT>(&reinterpret_cast<cx<T>*>(as_global(dst))[base - (long long)c * P.out.cstride + j], scl(v, s));
         }
       }
     }

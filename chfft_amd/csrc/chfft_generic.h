@@ -62,7 +62,7 @@ __device__ __forceinline__ void lds_pass(cx<T>* lds, const TMap& m, int L, int L
 
 template <typename T>
 __device__ void run_passes(cx<T>* lds, const TMap& m, const SeqDesc& s, int Lp) {
-  const cx<T>* tw = reinterpret_cast<const cx<T>*>(s.tw);
+  const cx<T>* tw = reinterpret_cast<const cx<T>*>(as_global(s.tw));
   for (int q = 0; q < s.npass; q++) {
     const PassDesc& pd = s.pass[q];
     switch (pd.radix) {
@@ -82,7 +82,7 @@ template <typename T, int LD>
 __device__ void load_stage(const KParams& P, const TMap& m, const void* src, cx<T>* lds, int g0, int nt) {
   const int t = P.colmajor_ld ? m.tc : m.tr;
   const int u = P.colmajor_ld ? m.uc : m.ur;
-  const cx<T>* chirp = reinterpret_cast<const cx<T>*>(P.chirp);
+  const cx<T>* chirp = reinterpret_cast<const cx<T>*>(as_global(P.chirp));
   cx<T>* row = lds + t * P.Lp;
   if (t < nt) {
     const Loader<T, LD> ld = make_loader<T, LD>(P, src, g0 + t);
@@ -122,7 +122,7 @@ __device__ void load_stage(const KParams& P, const TMap& m, const void* src, cx<
 // that the inverse transform reuses the forward passes.
 template <typename T>
 __device__ void mid_stage(const KParams& P, const TMap& m, cx<T>* lds, int g0) {
-  const cx<T>* K = reinterpret_cast<const cx<T>*>(P.kspec);
+  const cx<T>* K = reinterpret_cast<const cx<T>*>(as_global(P.kspec));
   const int c = (g0 + m.tr) % P.in.ncols;
   const long long kb = (long long)c * P.kc;
   cx<T>* row = lds + m.tr * P.Lp;
@@ -133,7 +133,7 @@ __device__ void mid_stage(const KParams& P, const TMap& m, cx<T>* lds, int g0) {
 // Fused Bluestein de-chirp (chirpz.rs:127-145 with the conjugation undone).
 template <typename T>
 __device__ void dechirp_stage(const KParams& P, const TMap& m, cx<T>* lds) {
-  const cx<T>* chirp = reinterpret_cast<const cx<T>*>(P.chirp);
+  const cx<T>* chirp = reinterpret_cast<const cx<T>*>(as_global(P.chirp));
   cx<T>* row = lds + m.tr * P.Lp;
   for (int n = m.ur; n < P.L; n += m.tpt) row[lx(n)] = cj(row[lx(n)]) * chirp[n];
   __syncthreads();

@@ -56,23 +56,61 @@ namespace chfft {
 
 // Each tile path is a function of its own (not inlined), so that the
 // register allocation of one shape does not add to the others' or to the
-// persistent loop's.
-#define CHFFT_SPEC_FN(ID, L, TPT, TPB, COL, LD, ST, MODE, ...)                                                \
-  __device__ __attribute__((noinline)) void spec_tile_##ID(const KParams& Q, const void* src, void* dst, int g0, \
-                                                           double scale, cx<float>* lds) {                     \
-    sfft_tile<float, L, TPT, TPB, COL, LD, ST, MODE, false, Rad<__VA_ARGS__>>(Q, src, dst, g0, scale, lds);   \
+// persistent loop's. Its arguments arrive in vector registers as generic
+// pointers; the tile re-derives them (uniform_global, uniform_desc) so that
+// the descriptor is read with scalar loads and the data and tables with
+// global instructions rather than flat ones, which also count against the
+// LDS wait counter and would make every LDS-only wait before one of the
+// tile's barriers drain its global loads in flight.
+__device__ __forceinline__ uintptr_t uniform_bits(uintptr_t v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return ((uintptr_t)hi << 32) | lo;
+}
+// A workgroup-uniform pointer to global memory, as a global-address-space
+// pointer.
+template <typename T>
+__device__ __forceinline__ T* uniform_global(T* p) {
+  return as_global((T*)uniform_bits((uintptr_t)p));
+}
+// The descriptor at the workgroup-uniform address Qd (read-only during the
+// launch), read through the constant address space, with its table
+// pointers marked global.
+__device__ __forceinline__ KParams uniform_desc(const KParams* Qd) {
+  static_assert(sizeof(KParams) % 4 == 0, "the descriptor is copied in 32-bit words");
+  typedef __attribute__((address_space(4))) const unsigned CW;
+  const CW* w = (const CW*)uniform_bits((uintptr_t)Qd);
+  KParams Q;
+  unsigned* q = reinterpret_cast<unsigned*>(&Q);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(KParams) / 4); i++) q[i] = w[i];
+  Q.seq.tw = uniform_global(Q.seq.tw);
+  Q.chirp = uniform_global(Q.chirp);
+  Q.kspec = uniform_global(Q.kspec);
+  Q.tab_a = uniform_global(Q.tab_a);
+  Q.tab_b = uniform_global(Q.tab_b);
+  Q.tw_rec = uniform_global(Q.tw_rec);
+  return Q;
+}
+
+#define CHFFT_SPEC_FN(ID, L, TPT, TPB, COL, LD, ST, MODE, ...)                                                 \
+  __device__ __attribute__((noinline)) void spec_tile_##ID(const KParams* Qd, const void* src, void* dst,      \
+                                                           int g0, double scale, cx<float>* lds) {             \
+    const KParams Q = uniform_desc(Qd);                                                                      \
+    sfft_tile<float, L, TPT, TPB, COL, LD, ST, MODE, false, Rad<__VA_ARGS__>>(                                \
+        Q, uniform_global(src), uniform_global(dst), __builtin_amdgcn_readfirstlane(g0), scale, lds);           \
   }
 CHFFT_MULTI_SPECS(CHFFT_SPEC_FN)
 #undef CHFFT_SPEC_FN
 
 template <typename T>
-__device__ __forceinline__ bool spec_tile(const KParams& Q, const void* src, void* dst, int g0, double scale,
-                                          cx<T>* lds) {
+__device__ __forceinline__ bool spec_tile(int spec, const KParams* Qd, const void* src, void* dst, int g0,
+                                          double scale, cx<T>* lds) {
   if constexpr (sizeof(T) == 4) {
-    switch (Q.spec) {
+    switch (spec) {
 #define CHFFT_SPEC_CASE(ID, ...)                    \
   case ID:                                         \
-    spec_tile_##ID(Q, src, dst, g0, scale, lds);   \
+    spec_tile_##ID(Qd, src, dst, g0, scale, lds);  \
     return true;
       CHFFT_MULTI_SPECS(CHFFT_SPEC_CASE)
 #undef CHFFT_SPEC_CASE
@@ -102,7 +140,10 @@ __device__ __forceinline__ bool grid_barrier(unsigned* sync, unsigned target, un
     __hip_atomic_fetch_add(&sync[0], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     int good = 1;
     const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-    while (__hip_atomic_load(&sync[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    // relaxed polls: an acquire load would invalidate this XCD's L2 on every
+    // poll, under the workgroups still working on the phase; the fence after
+    // the wait orders what follows
+    while (__hip_atomic_load(&sync[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(8);
       if (__builtin_amdgcn_s_memrealtime() - t0 > ticks) {
         good = 0;
@@ -170,7 +211,7 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? kMultiThreadsF64 : kMultiThre
         const double psc = last ? scale : 1.0;
         for (int g = U.g0; g < U.g1; g += Q.tpb) {
           const int nt = min(Q.tpb, U.g1 - g);
-          if (Q.spec && spec_tile<T>(Q, psrc, pdst, g, psc, lds)) {
+          if (Q.spec && spec_tile<T>(Q.spec, &Q, psrc, pdst, g, psc, lds)) {
             // register-resident tile of a registered shape
           } else if (m.kind == 1) {
             if (U.ph == 0)

@@ -2262,9 +2262,15 @@ int pow2_ceil_log(size_t n) {
 }
 
 // tpb for a phase of length L under a fixed workgroup of `block` threads:
-// column phases want 16 transforms (128-byte rows), all need tpt >= L/16.
+// the most transforms (up to 64) for which tpt >= L/16 and the tile fits in
+// LDS. Column phases want at least 16 (128-byte rows); beyond that, a wider
+// tile spreads each pass's barrier and twiddle-load latency over more points
+// (the multi-length kernel runs one workgroup per CU, with nothing else to
+// hide that latency: 16 -> 64 took the 10,000-length batch's smooth
+// four-step items from 2.90 to 2.11 ms on MI355X).
 int phase_tpb(size_t L, int block, bool colmajor, bool f64) {
-  int tpb = colmajor ? 16 : 16;
+  (void)colmajor;
+  int tpb = 64;
   const size_t esz = f64 ? 16 : 8;
   while (tpb > 1) {
     const int tpt = block / tpb;
