@@ -3,7 +3,8 @@
 // resident in LDS.
 //   1. load stage   global -> LDS, through a load operator (plain, chirp,
 //                   RFft/DCT/MDCT pre-pass), optional conjugation
-//   2. passes       radix-2/3/4/5/7/8/16 Stockham autosort passes LDS -> LDS,
+//   2. passes       radix-2/3/4/5/7/8/16 (and 10/12/15 where the kernel asks)
+//                   Stockham autosort passes LDS -> LDS,
 //                   inputs staged in registers between two barriers
 //   3. (Bluestein)  spectrum product, conjugate, second pass sequence,
 //                   de-chirp
@@ -60,7 +61,10 @@ __device__ __forceinline__ void lds_pass(cx<T>* lds, const TMap& m, int L, int L
   __syncthreads();
 }
 
-template <typename T>
+// PFA: the schedule may also hold the prime-factor radices 10, 12 and 15
+// (fewer passes for lengths with factors 3 and 5); only the kernels that
+// take such schedules compile those cases, which cost the others registers.
+template <typename T, bool PFA = false>
 __device__ void run_passes(cx<T>* lds, const TMap& m, const SeqDesc& s, int Lp) {
   const cx<T>* tw = reinterpret_cast<const cx<T>*>(as_global(s.tw));
   for (int q = 0; q < s.npass; q++) {
@@ -73,7 +77,16 @@ __device__ void run_passes(cx<T>* lds, const TMap& m, const SeqDesc& s, int Lp) 
       case 7: lds_pass<T, 7>(lds, m, s.L, Lp, pd, tw); break;
       case 8: lds_pass<T, 8>(lds, m, s.L, Lp, pd, tw); break;
       case 16: lds_pass<T, 16>(lds, m, s.L, Lp, pd, tw); break;
-      default: break;
+      default:
+        if constexpr (PFA) {
+          switch (pd.radix) {
+            case 10: lds_pass<T, 10>(lds, m, s.L, Lp, pd, tw); break;
+            case 12: lds_pass<T, 12>(lds, m, s.L, Lp, pd, tw); break;
+            case 15: lds_pass<T, 15>(lds, m, s.L, Lp, pd, tw); break;
+            default: break;
+          }
+        }
+        break;
     }
   }
 }
@@ -141,15 +154,15 @@ __device__ void dechirp_stage(const KParams& P, const TMap& m, cx<T>* lds) {
 
 // One LDS-resident batch of tpb transforms: load, passes, [Bluestein middle],
 // store. The store stage is not followed by a barrier.
-template <typename T, int LD, int ST>
+template <typename T, int LD, int ST, bool PFA = false>
 __device__ __forceinline__ void block_transform(const KParams& P, const void* src, void* dst, cx<T>* lds, int g0,
                                                 int nt, double scale) {
   const TMap m = make_map(P.tpb);
   load_stage<T, LD>(P, m, src, lds, g0, nt);
-  run_passes<T>(lds, m, P.seq, P.Lp);
+  run_passes<T, PFA>(lds, m, P.seq, P.Lp);
   if (P.has_mid) {
     mid_stage<T>(P, m, lds, g0);
-    run_passes<T>(lds, m, P.seq, P.Lp);
+    run_passes<T, PFA>(lds, m, P.seq, P.Lp);
     if (P.has_chirp) dechirp_stage<T>(P, m, lds);
   }
   store_stage<T, ST>(P, m, dst, lds, g0, nt, scale);

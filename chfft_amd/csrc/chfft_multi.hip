@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? kMultiThreadsF64 : kMultiThre
       const double n = (double)m.n;
       const double scale = scale_kind == 0 ? 1.0 : scale_kind == 1 ? 1.0 / sqrt(n) : 1.0 / n;
       if (m.kind == 0) {
-        block_transform<T, LD_PLAIN, ST_PLAIN>(descs[m.desc], src, dst, lds, 0, 1, scale);
+        block_transform<T, LD_PLAIN, ST_PLAIN, true>(descs[m.desc], src, dst, lds, 0, 1, scale);
         __syncthreads();
       } else if (m.kind == 3) {
         // length 1: returned unchanged (cfft1d.rs:213-214)
@@ -215,16 +215,16 @@ __global__ void __launch_bounds__(sizeof(T) == 8 ? kMultiThreadsF64 : kMultiThre
             // register-resident tile of a registered shape
           } else if (m.kind == 1) {
             if (U.ph == 0)
-              block_transform<T, LD_PLAIN, ST_TW>(Q, src, slot, lds, g, nt, 1.0);
+              block_transform<T, LD_PLAIN, ST_TW, true>(Q, src, slot, lds, g, nt, 1.0);
             else
-              block_transform<T, LD_PLAIN, ST_PLAIN>(Q, slot, dst, lds, g, nt, scale);
+              block_transform<T, LD_PLAIN, ST_PLAIN, true>(Q, slot, dst, lds, g, nt, scale);
           } else {
             if (U.ph == 0)
-              block_transform<T, LD_FS_CHIRP, ST_TW>(Q, src, slot, lds, g, nt, 1.0);
+              block_transform<T, LD_FS_CHIRP, ST_TW, true>(Q, src, slot, lds, g, nt, 1.0);
             else if (U.ph == 1)
-              block_transform<T, LD_PLAIN, ST_TW>(Q, slot, slot, lds, g, nt, 1.0);
+              block_transform<T, LD_PLAIN, ST_TW, true>(Q, slot, slot, lds, g, nt, 1.0);
             else
-              block_transform<T, LD_PLAIN, ST_FS_DECHIRP>(Q, slot, dst, lds, g, nt, scale);
+              block_transform<T, LD_PLAIN, ST_FS_DECHIRP, true>(Q, slot, dst, lds, g, nt, scale);
           }
           __syncthreads();  // LDS reuse by the next tile
         }

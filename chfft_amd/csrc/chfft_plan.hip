@@ -2328,10 +2328,12 @@ static int multi_create(const size_t* lens, size_t count, int precision, const T
   std::map<std::vector<int>, int> seq_of_sched;  // key: {L, compact?, radices...}
   long long arena = 0;
   // pass sequence of length L with radix schedule `rad` (default: the
-  // runtime-radix schedule); `compact`: the length-specialised tiles' table
+  // runtime-radix schedule with the prime-factor radices 10, 12 and 15, which
+  // the multi-length kernel's generic passes take: fewer passes for lengths
+  // with factors 3 and 5); `compact`: the length-specialised tiles' table
   // layout, the compact tables (two-level where large) after the full ones
   auto seq_for = [&](size_t L, std::vector<int> rad = {}, bool compact = false) {
-    if (rad.empty()) rad = radices_for(L);
+    if (rad.empty()) rad = radices_for(L, true);
     std::vector<int> key = {(int)L, compact ? 1 : 0};
     key.insert(key.end(), rad.begin(), rad.end());
     auto it = seq_of_sched.find(key);
