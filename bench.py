@@ -23,6 +23,7 @@ through a HIP graph), one record per case with:
     min(cores, 16) threads and on 1 core.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only ID ...] [--copy-bytes B]
+                [--uneven-class all|fused|smooth4|blue4_small|blue4_big]
 """
 import argparse
 import ctypes
@@ -57,7 +58,37 @@ def parse():
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only time a plain device-to-device copy of this many bytes (the reference kernel for "
                          "the HBM traffic counters and the achievable copy bandwidth)")
+    ap.add_argument("--uneven-class", default=None,
+                    choices=("all", "fused", "smooth4", "blue4_small", "blue4_big"),
+                    help="only time the FP32 multi-length plan over one size class of the 10,000-length uneven "
+                         "batch (for per-class kernel counters)")
     return ap.parse_args()
+
+
+def uneven_class_timing(torch, name, reps=5):
+    """One persistent multi-length launch over a size class of the uneven
+    batch, timed with HIP events over `reps` back-to-back executions."""
+    from chfft_amd import MultiFFT
+    from chfft_amd.workloads import uneven_class, uneven_lengths
+    lens = uneven_class(uneven_lengths(), name)
+    fft = MultiFFT(lens, np.float32)
+    x = torch.empty(fft.total, dtype=torch.complex64, device="cuda")
+    x.view(torch.float32).normal_()
+    y = torch.empty_like(x)
+    st = torch.cuda.current_stream().cuda_stream
+    fft.plan.exec_ptr(x.data_ptr(), y.data_ptr(), "forward", st)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fft.plan.exec_ptr(x.data_ptr(), y.data_ptr(), "forward", st)
+    e1.record()
+    torch.cuda.synchronize()
+    fft.plan.check()
+    ms = e0.elapsed_time(e1) / reps
+    return {"uneven_class": name, "transforms": len(lens), "points": fft.total, "ms": round(ms, 4),
+            "schedule": fft.plan.describe().strip().splitlines()[0]}
 
 
 def copy_calibration(torch, nbytes, reps=10):
@@ -540,6 +571,9 @@ def main():
     torch.cuda.init()
     if args.copy_bytes:
         print(json.dumps(copy_calibration(torch, args.copy_bytes)), flush=True)
+        return
+    if args.uneven_class:
+        print(json.dumps(uneven_class_timing(torch, args.uneven_class)), flush=True)
         return
     t0 = time.time()
     cpu_note = None

@@ -57,3 +57,30 @@ def uneven_lengths(count=10000, lo=16, hi=1 << 20, cap=1 << 28, seed=0x5EED0005)
         out.append(n)
         total += n
     return out
+
+
+def _smooth7(n):
+    for p in (2, 3, 5, 7):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+UNEVEN_CLASSES = ("all", "fused", "smooth4", "blue4_small", "blue4_big")
+
+
+def uneven_class(lens, name):
+    """The lengths of one size class of an uneven batch, by how the FP32
+    multi-length plan runs them: "fused" (7-smooth up to 16384 points, other
+    lengths up to 8192: one workgroup each), "smooth4" (7-smooth above 16384:
+    four-step), "blue4_small" / "blue4_big" (four-step Bluestein, N up to /
+    above 65536), or "all"."""
+    if name == "all":
+        return list(lens)
+    sel = {
+        "fused": lambda n: n <= 16384 if _smooth7(n) else n <= 8192,
+        "smooth4": lambda n: _smooth7(n) and n > 16384,
+        "blue4_small": lambda n: not _smooth7(n) and 8192 < n <= 65536,
+        "blue4_big": lambda n: not _smooth7(n) and n > 65536,
+    }[name]
+    return [n for n in lens if sel(n)]
