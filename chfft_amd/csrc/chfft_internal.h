@@ -120,7 +120,10 @@ struct KParams {
   // four-step twiddles by recurrence (length-specialised ST_TW stores): FP64
   // table [r][c] of W^(c r) for r < K0 (the kernel's base rows, K0 = its
   // last-pass butterflies per thread x threads per transform) and, in row K0,
-  // the step W^(c p) (p: the last pass's span); null: the full table tab_a
+  // the step W^(c p) (p: the last pass's span); null: the full table tab_a.
+  // For ST_TW2 stores (the multi-length kernel's tiles): FP64 two-level tables
+  // [W^i, i < 2^twsh][W^(i 2^twsh)] from which the tile forms each base and
+  // step; null: the FP32 two-level tables tab_a, tab_b, twice per element
   const void* tw_rec;
   int spec;         // multi-length kernel: register-resident tile path of this descriptor (0: generic)
   int nt_out;       // 1: the launch writes the exec's final output (no later launch reads it): a registered

@@ -1327,6 +1327,7 @@ struct chfft_plan_s {
   size_t total = 0;  // complex elements over all transforms
   int nitems = 0;
   DevBuf arena, items_dev, units_dev, descs_fwd, descs_bwd, sync_dev;
+  DevBuf arena64;  // FP64 two-level twiddle tables of the register-resident tiles' four-step stores
   int unit_ends[kMultiPhases] = {0, 0, 0};  // units of phases 0..2 end here
   size_t multi_lds = 0;
   int multi_grid = 0, multi_occ = 1;
@@ -2251,6 +2252,7 @@ struct LenInfo {
   // widths, SeqTab indices of their schedules and LDS bytes
   int spec[3] = {0, 0, 0}, spec_tpb[3] = {0, 0, 0}, spec_seq[3] = {-1, -1, -1};
   size_t spec_lds[3] = {0, 0, 0};
+  long long rec64 = -1;  // FP64 two-level tables in arena64 (register-resident four-step stores)
   int desc = 0;  // first descriptor of this length (per direction)
   int ndesc = 0;
 };
@@ -2542,6 +2544,43 @@ static int multi_create(const size_t* lens, size_t count, int precision, const T
     }
   }
 
+  // ---- FP64 two-level tables for the register-resident tiles' four-step
+  // stores, one pair per distinct M: [W_M^i, i < 2^twsh] then
+  // [W_M^(i 2^twsh), i < ceil(M / 2^twsh)]; the tiles take W^(c k) for a
+  // butterfly's first output and the step between its outputs from them and
+  // run the rest by an FP64 recurrence, each twiddle rounded once
+  {
+    std::map<size_t, long long> rec_of_m;
+    std::vector<RootJob> jobs64;
+    long long a64 = 0;
+    for (auto& kv : info) {
+      LenInfo& L = kv.second;
+      if (L.kind != 1 && L.kind != 2) continue;
+      if (!(L.spec[0] || L.spec[1])) continue;
+      auto it = rec_of_m.find(L.M);
+      if (it != rec_of_m.end()) {
+        L.rec64 = it->second;
+        continue;
+      }
+      const long long nlo = 1ll << L.twsh, nhi = (long long)((L.M + nlo - 1) >> L.twsh);
+      jobs64.push_back({a64, nlo, (long long)L.M, 1, 3, 0});
+      jobs64.push_back({a64 + nlo, nhi, (long long)L.M, 1, 2, L.twsh});
+      L.rec64 = rec_of_m[L.M] = a64;
+      a64 += nlo + nhi;
+    }
+    if (!jobs64.empty()) {
+      if ((rc = dev_alloc(p->arena64, (size_t)a64 * 16))) return rc;
+      DevBuf jd;
+      if ((rc = dev_alloc(jd, jobs64.size() * sizeof(RootJob)))) return rc;
+      if (hipMemcpy(jd.p, jobs64.data(), jobs64.size() * sizeof(RootJob), hipMemcpyHostToDevice) != hipSuccess)
+        return CHFFT_ERR_HIP;
+      if (gen_roots(true, reinterpret_cast<const RootJob*>(jd.p), (int)jobs64.size(), p->arena64.p, nullptr) !=
+          hipSuccess)
+        return CHFFT_ERR_HIP;
+      if (hipDeviceSynchronize() != hipSuccess) return CHFFT_ERR_HIP;
+    }
+  }
+
   // ---- descriptors (per distinct length, both directions) and items
   const char* base = reinterpret_cast<const char*>(p->arena.p);
   auto at = [&](long long off) { return (const void*)(base + (size_t)off * esz); };
@@ -2631,6 +2670,8 @@ static int multi_create(const size_t* lens, size_t count, int precision, const T
             Q.tpb = L.spec_tpb[ph];
             Q.seq = seqs[L.spec_seq[ph]].seq;
             Q.seq.tw = at(seqs[L.spec_seq[ph]].off);
+            if (ph < 2 && L.rec64 >= 0)  // the phases whose tiles store through ST_TW2
+              Q.tw_rec = reinterpret_cast<const char*>(p->arena64.p) + (size_t)L.rec64 * 16;
             lds = std::max(lds, L.spec_lds[ph]);
           }
         D.push_back(A);

@@ -990,12 +990,27 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
       const int trn = P.out.ncols;
       cx<double> wst = mk(1.0, 0.0);
       if (REC && trc) wst = trc[BL * TPT * trn + co];
+      // ST_TW2 by recurrence where the plan has FP64 two-level tables (tw_rec:
+      // [W^i, i < 2^twsh][W^(i 2^twsh)]): W^(c k) for each butterfly's first
+      // output k = u + b TPT and the step W^(c PLAST) from two lookups each, the
+      // rest by FP64 products, each twiddle rounded once
+      constexpr bool REC2 = ST == ST_TW2;
+      const cx<double>* t2 = REC2 ? reinterpret_cast<const cx<double>*>(P.tw_rec) : nullptr;
+      if (REC2 && t2) {
+        const unsigned ms = (unsigned)co * (unsigned)PLAST;
+        wst = t2[(1u << sh) + (ms >> sh)] * t2[ms & lo];
+      }
+      const bool rec = (REC && trc) || (REC2 && t2);
 #pragma unroll
       for (int b = 0; b < BL; b++) {
         const int i = u + b * TPT;
         if (PL::FULL(NP - 1) || i < PL::NB(NP - 1)) {
           cx<double> wd = mk(1.0, 0.0);
           if (REC && trc) wd = trc[i * trn + co];
+          if (REC2 && t2) {
+            const unsigned m0 = (unsigned)co * (unsigned)i;
+            wd = t2[(1u << sh) + (m0 >> sh)] * t2[m0 & lo];
+          }
 #pragma unroll
           for (int j = 0; j < RL; j++) {
             const int rel = b * TPT + j * PLAST;  // k = u + rel
@@ -1009,13 +1024,17 @@ __device__ __forceinline__ void sfft_body(const KParams& P, const void* src, voi
                 else
                   v = v * ld_off<cx<T>>(reinterpret_cast<const char*>(P.tab_a) + (unsigned)rel * tk, toff);
               } else if constexpr (ST == ST_TW2) {
-                const unsigned m = mu + (unsigned)co * (unsigned)rel;
-                v = v * (ta[m >> sh] * tb[m & lo]);
+                if (t2) {
+                  v = v * mk((T)wd.re, (T)wd.im);
+                } else {
+                  const unsigned m = mu + (unsigned)co * (unsigned)rel;
+                  v = v * (ta[m >> sh] * tb[m & lo]);
+                }
               }
               if (cjo) v = cj(v);
               st_off<cx<T>, NT>(ob + (unsigned)rel * es, voff, scl(v, s));
             }
-            if (REC && trc && j + 1 < RL) wd = wd * wst;
+            if (rec && j + 1 < RL) wd = wd * wst;
           }
         }
       }
