@@ -542,13 +542,32 @@ __device__ void store_stage(const KParams& P, const TMap& m, void* dst, const cx
     const long long base = io_base(P.out, g0 + t, &c);
     cx<T>* out = reinterpret_cast<cx<T>*>(as_global(dst)) + base;
     const long long es = P.out.estride;
+    // two-level four-step twiddles by FP64 recurrence where the plan has FP64
+    // two-level tables (tw_rec: [W^i, i < 2^twsh][W^(i 2^twsh)]): W^(c u) and
+    // the step W^(c tpt) from two lookups each (the step only where a second
+    // output exists, so that c tpt < c L <= the table's length), each twiddle
+    // rounded once
+    const cx<double>* t2 = (ST == ST_TW && P.twsh) ? reinterpret_cast<const cx<double>*>(as_global(P.tw_rec)) : nullptr;
+    cx<double> wd = mk(1.0, 0.0), wst = mk(1.0, 0.0);
+    if (ST == ST_TW && t2) {
+      const unsigned sh = (unsigned)P.twsh, lo = (1u << sh) - 1u;
+      const unsigned m0 = (unsigned)c * (unsigned)u;
+      wd = t2[(1u << sh) + (m0 >> sh)] * t2[m0 & lo];
+      if (u + m.tpt < P.L) {
+        const unsigned ms = (unsigned)c * (unsigned)m.tpt;
+        wst = t2[(1u << sh) + (ms >> sh)] * t2[ms & lo];
+      }
+    }
     for (int k = u; k < P.L; k += m.tpt) {
       cx<T> v = row[lx(k)];
       if constexpr (ST == ST_PLAIN) {
         if (P.conj_out) v = cj(v);
         gput<NT>(&out[k * es], scl(v, s));
       } else if constexpr (ST == ST_TW) {
-        if (P.twsh) {
+        if (t2) {
+          v = v * mk((T)wd.re, (T)wd.im);
+          wd = wd * wst;
+        } else if (P.twsh) {
           // two-level table: W^m = W^(m_hi << sh) * W^(m_lo), both rounded once
           const int mm = c * k;
           v = v * (reinterpret_cast<const cx<T>*>(as_global(P.tab_a))[mm >> P.twsh] *

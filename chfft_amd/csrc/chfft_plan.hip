@@ -2544,11 +2544,11 @@ static int multi_create(const size_t* lens, size_t count, int precision, const T
     }
   }
 
-  // ---- FP64 two-level tables for the register-resident tiles' four-step
-  // stores, one pair per distinct M: [W_M^i, i < 2^twsh] then
-  // [W_M^(i 2^twsh), i < ceil(M / 2^twsh)]; the tiles take W^(c k) for a
-  // butterfly's first output and the step between its outputs from them and
-  // run the rest by an FP64 recurrence, each twiddle rounded once
+  // ---- FP64 two-level tables for the four-step twiddle stores, one pair
+  // per distinct M: [W_M^i, i < 2^twsh] then [W_M^(i 2^twsh), i <
+  // ceil(M / 2^twsh)]; the stores take W^(c k) for a thread's first output
+  // and the step between its outputs from them and run the rest by an FP64
+  // recurrence, each twiddle rounded once
   {
     std::map<size_t, long long> rec_of_m;
     std::vector<RootJob> jobs64;
@@ -2556,7 +2556,6 @@ static int multi_create(const size_t* lens, size_t count, int precision, const T
     for (auto& kv : info) {
       LenInfo& L = kv.second;
       if (L.kind != 1 && L.kind != 2) continue;
-      if (!(L.spec[0] || L.spec[1])) continue;
       auto it = rec_of_m.find(L.M);
       if (it != rec_of_m.end()) {
         L.rec64 = it->second;
@@ -2662,6 +2661,14 @@ static int multi_create(const size_t* lens, size_t count, int precision, const T
         }
         lds = std::max(lds, (size_t)ta * A.Lp * esz);
         lds = std::max(lds, (size_t)tb * B.Lp * esz);
+        // the four-step twiddle stores (columns; Bluestein rows) run by FP64
+        // recurrence from the FP64 two-level tables, in the generic stores
+        // and in the register-resident tiles alike
+        if (L.rec64 >= 0) {
+          const void* t64 = reinterpret_cast<const char*>(p->arena64.p) + (size_t)L.rec64 * 16;
+          A.tw_rec = t64;
+          if (L.kind == 2) B.tw_rec = t64;
+        }
         KParams* Qs[3] = {&A, &B, &C};
         for (int ph = 0; ph < 3; ph++)
           if (L.spec[ph]) {
@@ -2670,8 +2677,6 @@ static int multi_create(const size_t* lens, size_t count, int precision, const T
             Q.tpb = L.spec_tpb[ph];
             Q.seq = seqs[L.spec_seq[ph]].seq;
             Q.seq.tw = at(seqs[L.spec_seq[ph]].off);
-            if (ph < 2 && L.rec64 >= 0)  // the phases whose tiles store through ST_TW2
-              Q.tw_rec = reinterpret_cast<const char*>(p->arena64.p) + (size_t)L.rec64 * 16;
             lds = std::max(lds, L.spec_lds[ph]);
           }
         D.push_back(A);
