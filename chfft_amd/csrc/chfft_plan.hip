@@ -381,7 +381,8 @@ std::vector<std::vector<int>> radix_variants(size_t L) {
 struct Tune {
   int generic = 0;      // 1: runtime-radix kernels only (no length-specialised kernel)
   int order = 0;        // radix_variants() index of each runtime-radix sequence (where it has one)
-  int tpb = 0, tpt = 0; // fused launches: transforms per workgroup, threads per transform
+  int tpb = 0, tpt = 0; // fused launches: transforms per workgroup, threads per transform; multi-length
+                        // plans (tpb only): the most transforms of a generic four-step tile (default 64)
   long long n1 = 0;     // four-step split N = n1 x n2
   long long chunk = 0;  // transforms per chunk of the split four-step (-1: the whole batch)
   int sched = 0;        // 1: one persistent four-step launch; 2: split four-step launches, also where
@@ -2264,15 +2265,16 @@ int pow2_ceil_log(size_t n) {
 }
 
 // tpb for a phase of length L under a fixed workgroup of `block` threads:
-// the most transforms (up to 64) for which tpt >= L/16 and the tile fits in
+// the most transforms (up to maxtpb, 64 unless tuned) for which tpt >= L/16
+// and the tile fits in
 // LDS. Column phases want at least 16 (128-byte rows); beyond that, a wider
 // tile spreads each pass's barrier and twiddle-load latency over more points
 // (the multi-length kernel runs one workgroup per CU, with nothing else to
 // hide that latency: 16 -> 64 took the 10,000-length batch's smooth
 // four-step items from 2.90 to 2.11 ms on MI355X).
-int phase_tpb(size_t L, int block, bool colmajor, bool f64) {
+int phase_tpb(size_t L, int block, bool colmajor, bool f64, int maxtpb) {
   (void)colmajor;
-  int tpb = 64;
+  int tpb = maxtpb;
   const size_t esz = f64 ? 16 : 8;
   while (tpb > 1) {
     const int tpt = block / tpb;
@@ -2297,12 +2299,15 @@ int lp_for(size_t L, int tpb) {
 // The multi-length plan under the planner overrides `t`: `generic` keeps
 // every four-step phase on the runtime-radix tiles, `grid` sets the
 // persistent launch's workgroup count (at most every workgroup that fits at
-// once), `unit` the points per work unit of a four-step phase.
+// once), `unit` the points per work unit of a four-step phase, `tpb` the
+// most transforms of a generic four-step tile (a power of two up to 128).
 static int multi_create(const size_t* lens, size_t count, int precision, const Tune& t, chfft_plan* plan) {
   if (!plan) return CHFFT_ERR_INVALID_ARGUMENT;
   *plan = nullptr;
-  if (t.tpb || t.tpt || t.order || t.n1 || t.chunk || t.sched || (t.unit && t.unit < 1024))
+  if ((t.tpb & (t.tpb - 1)) || t.tpb < 0 || t.tpb > 128 || t.tpt || t.order || t.n1 || t.chunk || t.sched ||
+      (t.unit && t.unit < 1024))
     return CHFFT_ERR_INVALID_ARGUMENT;
+  const int maxtpb = t.tpb ? t.tpb : 64;
   if (check_prec(precision)) return CHFFT_ERR_INVALID_ARGUMENT;
   if (count > 0 && !lens) return CHFFT_ERR_INVALID_ARGUMENT;
   size_t total = 0;
@@ -2613,7 +2618,7 @@ static int multi_create(const size_t* lens, size_t count, int precision, const T
         lds = std::max(lds, (size_t)P.Lp * esz);
         D.push_back(P);
       } else {
-        const int ta = phase_tpb(L.n1, block, true, f64), tb = phase_tpb(L.n2, block, L.kind == 1, f64);
+        const int ta = phase_tpb(L.n1, block, true, f64, maxtpb), tb = phase_tpb(L.n2, block, L.kind == 1, f64, maxtpb);
         KParams A{}, B{}, C{};
         A.seq = seqs[L.s1].seq;
         A.seq.tw = at(seqs[L.s1].off);
@@ -2867,7 +2872,8 @@ std::vector<size_t> alt_splits(size_t total, size_t n1, bool f64, size_t maxn) {
 // the fewest threads per transform, half / double the threads) on the
 // runtime-radix kernel. Four-step lengths: the persistent schedule, the
 // runtime-radix kernels, chunk sizes, other splits and radix orders.
-// Multi-length plans: workgroup counts, work-unit sizes, runtime-radix tiles.
+// Multi-length plans: workgroup counts, work-unit sizes, generic tile widths,
+// runtime-radix tiles.
 std::vector<Tune> tune_candidates(const chfft_plan_s* p) {
   std::vector<Tune> c(1);
   if (p->kind == K_MULTI) {
@@ -2879,6 +2885,11 @@ std::vector<Tune> tune_candidates(const chfft_plan_s* p) {
         t.unit = u;
         if (t.grid || t.unit) c.push_back(t);
       }
+    for (int w : {16, 32}) {  // narrower generic four-step tiles than the default 64
+      Tune t;
+      t.tpb = w;
+      c.push_back(t);
+    }
     if (!p->f64) {
       Tune t;
       t.generic = 1;

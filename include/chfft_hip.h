@@ -155,8 +155,9 @@ int chfft_plan_check(chfft_plan plan, void* stream);
    kernel; a four-step length the persistent schedule, the runtime-radix
    kernels, chunk sizes, other splits N = n1 x n2 and radix orders (a
    Bluestein length fused into one workgroup: the four-step schedules); a
-   multi-length plan workgroup counts, work-unit sizes and the runtime-radix
-   tiles. CFft1D and multi-length plans; other kinds return 0. */
+   multi-length plan workgroup counts, work-unit sizes, generic tile widths
+   and the runtime-radix tiles. CFft1D and multi-length plans; other kinds
+   return 0. */
 int chfft_plan_autotune(chfft_plan plan, void* in, void* out, int reps, void* stream);
 
 /* Planner overrides (no counterpart in the reference): rebuilds a CFft1D or
@@ -166,7 +167,9 @@ int chfft_plan_autotune(chfft_plan plan, void* in, void* out, int reps, void* st
      order=k     radix schedule k of each runtime-radix sequence (0 = the
                  planner's; fused lengths: an order the length has)
      tpb=, tpt=  fused length: transforms per workgroup, threads per
-                 transform (threads per transform >= length / 16)
+                 transform (threads per transform >= length / 16);
+                 multi-length (tpb only): the most transforms of a generic
+                 four-step tile, a power of two up to 128 (default 64)
      n1=         four-step split N = n1 x n2 (7-smooth factors one
                  workgroup holds)
      chunk=      four-step: transforms per chunk (-1 = the whole batch)

@@ -95,6 +95,7 @@ def test_multi_candidates_pass_parity():
     pick = sorted({0, len(lens) // 2, len(lens) - 1, int(np.argmax(lens))})
     xs = x.cpu().numpy()
     refs = {i: np.fft.fft(xs[offs[i]:offs[i + 1]].astype(np.complex128)) for i in pick}
+    assert any(s.startswith("tpb=") for _, s, _ in cands)  # generic tile widths are among them
     for idx, s, ms in cands:
         p.tune(s)
         p.exec(x, y, "forward")
@@ -103,6 +104,15 @@ def test_multi_candidates_pass_parity():
         for i in pick:
             e = rel_l2(ys[offs[i]:offs[i + 1]], refs[i])
             assert e <= tol_f32(lens[i]), f"[{idx}] {s}: item {i} (N={lens[i]}) rel-L2 {e:.3e}"
+    # generic tile widths: powers of two up to 128 only; the plan keeps working
+    for bad in ("tpb=3", "tpb=256", "tpt=4"):
+        with pytest.raises(ChfftError):
+            p.tune(bad)
+    p.exec(x, y, "forward")
+    p.check()
+    ys = y.cpu().numpy()
+    for i in pick:
+        assert rel_l2(ys[offs[i]:offs[i + 1]], refs[i]) <= tol_f32(lens[i])
 
 
 def test_invalid_specs_leave_the_plan_working():
