@@ -16,10 +16,15 @@ namespace chfft {
 //   rows of four-step Bluestein items: spectrum product and inverse rows,
 //   two-level twiddle on the store (MODE 2, ST_TW2);
 //   columns of four-step Bluestein items: chirp on the load, two-level
-//   twiddle on the store; inverse columns with the de-chirp on the store.
+//   twiddle on the store; inverse columns with the de-chirp on the store;
+//   columns of smooth four-step items (plain load, two-level twiddle) and
+//   their rows (plain passes, transposed store through the LDS epilogue:
+//   the only plain row store of this kernel, so ST_PLAIN rows take it).
 // Lengths: the row and column lengths 2^a {1, 3, 5, 15} that the planner's
 // splits of M in (16384, 2^21] produce most often (the 10,000-length uneven
-// workload: 12.56 -> 9.95 ms on MI355X with the first 22 of these).
+// workload: 12.56 -> 9.95 ms on MI355X with the first 22 of these), and the
+// 16 column and 16 row lengths that carry most points of its smooth items
+// (66 % and 73 % of them).
 #define CHFFT_MULTI_SPECS(X)                                                         \
   X(1, 1024, 64, 16, false, LD_PLAIN, ST_TW2, 2, 16, 16, 4)                          \
   X(2, 768, 64, 16, false, LD_PLAIN, ST_TW2, 2, 16, 16, 3)                           \
@@ -52,7 +57,39 @@ namespace chfft {
   X(28, 240, 16, 64, true, LD_PLAIN, ST_FS_DECHIRP, 0, 16, 5, 3)                     \
   X(29, 192, 16, 64, true, LD_PLAIN, ST_FS_DECHIRP, 0, 16, 4, 3)                     \
   X(32, 160, 16, 64, true, LD_PLAIN, ST_FS_DECHIRP, 0, 8, 4, 5)                      \
-  X(33, 128, 16, 64, true, LD_PLAIN, ST_FS_DECHIRP, 0, 16, 8)
+  X(33, 128, 16, 64, true, LD_PLAIN, ST_FS_DECHIRP, 0, 16, 8)                        \
+  X(40, 125, 16, 64, true, LD_PLAIN, ST_TW2, 0, 5, 5, 5)                             \
+  X(41, 150, 16, 64, true, LD_PLAIN, ST_TW2, 0, 10, 15)                              \
+  X(42, 225, 16, 64, true, LD_PLAIN, ST_TW2, 0, 15, 15)                              \
+  X(43, 243, 16, 64, true, LD_PLAIN, ST_TW2, 0, 3, 3, 3, 3, 3)                       \
+  X(44, 250, 16, 64, true, LD_PLAIN, ST_TW2, 0, 10, 5, 5)                            \
+  X(45, 270, 32, 32, true, LD_PLAIN, ST_TW2, 0, 2, 15, 3, 3)                         \
+  X(46, 300, 32, 32, true, LD_PLAIN, ST_TW2, 0, 4, 15, 5)                            \
+  X(47, 324, 32, 32, true, LD_PLAIN, ST_TW2, 0, 12, 3, 3, 3)                         \
+  X(48, 360, 32, 32, true, LD_PLAIN, ST_TW2, 0, 8, 15, 3)                            \
+  X(49, 375, 32, 32, true, LD_PLAIN, ST_TW2, 0, 15, 5, 5)                            \
+  X(50, 400, 32, 32, true, LD_PLAIN, ST_TW2, 0, 16, 5, 5)                            \
+  X(51, 405, 32, 32, true, LD_PLAIN, ST_TW2, 0, 15, 3, 3, 3)                         \
+  X(52, 450, 32, 32, true, LD_PLAIN, ST_TW2, 0, 2, 15, 15)                           \
+  X(53, 480, 32, 32, true, LD_PLAIN, ST_TW2, 0, 8, 4, 15)                            \
+  X(54, 486, 32, 32, true, LD_PLAIN, ST_TW2, 0, 2, 3, 3, 3, 3, 3)                    \
+  X(55, 500, 32, 32, true, LD_PLAIN, ST_TW2, 0, 4, 5, 5, 5)                          \
+  X(60, 240, 16, 64, false, LD_PLAIN, ST_PLAIN, 0, 16, 15)                           \
+  X(61, 256, 16, 64, false, LD_PLAIN, ST_PLAIN, 0, 16, 16)                           \
+  X(62, 288, 32, 32, false, LD_PLAIN, ST_PLAIN, 0, 8, 4, 3, 3)                       \
+  X(63, 320, 32, 32, false, LD_PLAIN, ST_PLAIN, 0, 16, 4, 5)                         \
+  X(64, 375, 32, 32, false, LD_PLAIN, ST_PLAIN, 0, 15, 5, 5)                         \
+  X(65, 384, 32, 32, false, LD_PLAIN, ST_PLAIN, 0, 16, 8, 3)                         \
+  X(66, 400, 32, 32, false, LD_PLAIN, ST_PLAIN, 0, 16, 5, 5)                         \
+  X(67, 432, 32, 32, false, LD_PLAIN, ST_PLAIN, 0, 16, 3, 3, 3)                      \
+  X(68, 480, 32, 32, false, LD_PLAIN, ST_PLAIN, 0, 8, 4, 15)                         \
+  X(69, 512, 32, 32, false, LD_PLAIN, ST_PLAIN, 0, 16, 8, 4)                         \
+  X(70, 576, 64, 16, false, LD_PLAIN, ST_PLAIN, 0, 16, 12, 3)                        \
+  X(71, 625, 64, 16, false, LD_PLAIN, ST_PLAIN, 0, 5, 5, 5, 5)                       \
+  X(72, 640, 64, 16, false, LD_PLAIN, ST_PLAIN, 0, 16, 8, 5)                         \
+  X(73, 720, 64, 16, false, LD_PLAIN, ST_PLAIN, 0, 16, 15, 3)                        \
+  X(74, 768, 64, 16, false, LD_PLAIN, ST_PLAIN, 0, 16, 16, 3)                        \
+  X(75, 800, 64, 16, false, LD_PLAIN, ST_PLAIN, 0, 16, 10, 5)
 
 // Each tile path is a function of its own (not inlined), so that the
 // register allocation of one shape does not add to the others' or to the
@@ -97,7 +134,7 @@ __device__ __forceinline__ KParams uniform_desc(const KParams* Qd) {
   __device__ __attribute__((noinline)) void spec_tile_##ID(const KParams* Qd, const void* src, void* dst,      \
                                                            int g0, double scale, cx<float>* lds) {             \
     const KParams Q = uniform_desc(Qd);                                                                      \
-    sfft_tile<float, L, TPT, TPB, COL, LD, ST, MODE, false, Rad<__VA_ARGS__>>(                                \
+    sfft_tile<float, L, TPT, TPB, COL, LD, ST, MODE, (ST == ST_PLAIN && !COL), Rad<__VA_ARGS__>>(            \
         Q, uniform_global(src), uniform_global(dst), __builtin_amdgcn_readfirstlane(g0), scale, lds);           \
   }
 CHFFT_MULTI_SPECS(CHFFT_SPEC_FN)
