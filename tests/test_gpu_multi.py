@@ -54,6 +54,30 @@ def test_multi_mixed_lengths(dtype, method):
     check_segments(fft, got, ref_segments(fft, x, method), f64)
 
 
+# column and row lengths of the register-resident tiles for smooth four-step
+# items (FP32): products of pairs of them, so that the planner's splits put
+# both phases of most items on those tiles. A wrong four-step twiddle keeps
+# the transform unitary (round trips and Parseval still hold), so every item
+# is compared with the reference directly.
+SMOOTH_COLS = [125, 150, 225, 243, 250, 270, 300, 324, 360, 375, 400, 405, 450, 480, 486, 500]
+SMOOTH_ROWS = [240, 256, 288, 320, 375, 384, 400, 432, 480, 512, 576, 625, 640, 720, 768, 800]
+# lengths whose preferred split is 225, 243, 300 or 450 columns, or 512 or 640
+# rows, which the pair products below do not reach
+SMOOTH_EXTRA = [54000, 58320, 76800, 165888, 168750, 307200]
+
+
+@pytest.mark.parametrize("method", ["forward", "backward"])
+def test_multi_smooth_tile_shapes(method):
+    lens = sorted({c * r for i, c in enumerate(SMOOTH_COLS)
+                   for r in (SMOOTH_ROWS[i], SMOOTH_ROWS[len(SMOOTH_ROWS) - 1 - i])} | set(SMOOTH_EXTRA))
+    fft = MultiFFT(lens, np.float32)
+    assert "four-step phases on register-resident tiles" in fft.plan.describe()
+    r = rng(53)
+    x = (r.standard_normal(fft.total) + 1j * r.standard_normal(fft.total)).astype(np.complex64)
+    got = getattr(fft, method)(x)
+    check_segments(fft, got, ref_segments(fft, x, method), False)
+
+
 def test_multi_graph_replay():
     # capture one exec in a HIP graph, replay it several times with eager work
     # in between; every replay must equal the eager result
