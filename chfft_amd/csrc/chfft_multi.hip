@@ -24,7 +24,8 @@ namespace chfft {
 // splits of M in (16384, 2^21] produce most often (the 10,000-length uneven
 // workload: 12.56 -> 9.95 ms on MI355X with the first 22 of these), and the
 // 16 column and 16 row lengths that carry most points of its smooth items
-// (66 % and 73 % of them).
+// (66 % and 73 % of them); Bluestein columns of 960 and 1024 and rows of
+// 1280 and 1536 (M up to 2^21).
 #define CHFFT_MULTI_SPECS(X)                                                         \
   X(1, 1024, 64, 16, false, LD_PLAIN, ST_TW2, 2, 16, 16, 4)                          \
   X(2, 768, 64, 16, false, LD_PLAIN, ST_TW2, 2, 16, 16, 3)                           \
@@ -89,7 +90,13 @@ namespace chfft {
   X(72, 640, 64, 16, false, LD_PLAIN, ST_PLAIN, 0, 16, 8, 5)                         \
   X(73, 720, 64, 16, false, LD_PLAIN, ST_PLAIN, 0, 16, 15, 3)                        \
   X(74, 768, 64, 16, false, LD_PLAIN, ST_PLAIN, 0, 16, 16, 3)                        \
-  X(75, 800, 64, 16, false, LD_PLAIN, ST_PLAIN, 0, 16, 10, 5)
+  X(75, 800, 64, 16, false, LD_PLAIN, ST_PLAIN, 0, 16, 10, 5)                        \
+  X(80, 1024, 64, 16, true, LD_FS_CHIRP, ST_TW2, 0, 16, 16, 4)                       \
+  X(81, 960, 64, 16, true, LD_FS_CHIRP, ST_TW2, 0, 16, 4, 15)                        \
+  X(82, 1024, 64, 16, true, LD_PLAIN, ST_FS_DECHIRP, 0, 16, 16, 4)                   \
+  X(83, 960, 64, 16, true, LD_PLAIN, ST_FS_DECHIRP, 0, 16, 4, 15)                    \
+  X(84, 1280, 128, 8, false, LD_PLAIN, ST_TW2, 2, 16, 16, 5)                         \
+  X(85, 1536, 128, 8, false, LD_PLAIN, ST_TW2, 2, 16, 8, 4, 3)
 
 // Each tile path is a function of its own (not inlined), so that the
 // register allocation of one shape does not add to the others' or to the
